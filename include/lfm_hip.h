@@ -1,0 +1,159 @@
+/*
+ * lfm_hip.h -- C ABI of the MI355X (gfx950) libFastMesh time step.
+ *
+ * The entry points replace, one for one, the virtual time-step methods of
+ * the reference's ISolver (api/cfdv0_solver.h:18-105) as Mesh<P,D>::solve
+ * (src/mesh_solver.cpp:474-740) drives them, plus the set-up that
+ * Mesh::initialize / Mesh::initializeSolver perform (src/mesh_reader.cpp:25,
+ * src/mesh_solver.cpp:32-125).  One handle holds both reference submeshes
+ * (boundary cells, then interior cells) of one partition in device memory.
+ *
+ * Conventions: every function returns 0 on success or a negative LFM_E*
+ * code and never throws; lfm_last_error() gives the message of the last
+ * failure on the calling thread.  Plain pointers and sizes only.  Host
+ * arrays are copied; the caller keeps ownership.  Field arrays are SoA
+ * [component][cell] in the ORIGINAL polyMesh cell order.  A handle must be
+ * used by one host thread at a time; its work is ordered on one HIP stream.
+ */
+#ifndef LFM_HIP_H
+#define LFM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LFM_OK 0
+#define LFM_EINVAL (-1)   /* bad argument or unsupported configuration      */
+#define LFM_EDEVICE (-2)  /* HIP runtime error                              */
+#define LFM_ENOMEM (-3)   /* device or host allocation failed / over budget */
+#define LFM_ESTATE (-4)   /* call out of order                              */
+#define LFM_ENONFINITE (-5) /* non-finite value detected in the state        */
+
+/* Boundary roles (init_boundary_conditions, src/cfd_v0.cpp:973-1005). */
+#define LFM_BC_UNSET 0    /* non-empty patch in no BC list: ghost q = 0     */
+#define LFM_BC_WALL 1
+#define LFM_BC_INLET 2
+#define LFM_BC_OUTLET 3
+#define LFM_BC_EMPTY 4
+#define LFM_BC_CYCLIC 5
+
+/* Precision of the device state. */
+#define LFM_PREC_FP64 64  /* plain FP64, the reference's double build      */
+#define LFM_PREC_FP32P 32 /* FP32 deviations from the freestream state     */
+
+/* Field selectors for lfm_get_field / lfm_set_field. */
+#define LFM_FIELD_Q 0       /* D+2 conservatives rho, rhoU, rhoE           */
+#define LFM_FIELD_DQ 1      /* D+2 delta_q                                 */
+#define LFM_FIELD_DUDX 2    /* D*D velocity gradient, row-major [i][j]     */
+#define LFM_FIELD_DTDX 3    /* D                                           */
+#define LFM_FIELD_TAUMC 4   /* D*D                                         */
+#define LFM_FIELD_SIGMAU 5  /* D                                           */
+#define LFM_FIELD_RHOGRAD 6 /* D   (solver 2 with minmod only)             */
+#define LFM_FIELD_PGRAD 7   /* D   (solver 2 with minmod only)             */
+#define LFM_FIELD_UGRAD 8   /* D*D (solver 2 with minmod only)             */
+#define LFM_FIELD_RES 9     /* D+2 per-cell residual of the last step      */
+#define LFM_FIELD_PRIM 16   /* D+2: p, U (D comps), T                      */
+
+/* OpenFOAM polyMesh in memory (constant/polyMesh/{points,faces,owner,
+ * neighbour,boundary}); patch roles resolved by the caller or by
+ * lfm_patch_role(). */
+typedef struct {
+    int32_t n_points;
+    const double *points;        /* [n_points*3]                          */
+    int32_t n_faces;
+    const int32_t *face_off;     /* [n_faces+1] CSR into face_pts         */
+    const int32_t *face_pts;
+    const int32_t *owner;        /* [n_faces]                             */
+    int32_t n_internal;
+    const int32_t *neighbour;    /* [n_internal]                          */
+    int32_t n_cells;
+    int32_t n_patches;
+    const int32_t *patch_start;
+    const int32_t *patch_size;
+    const int32_t *patch_role;   /* LFM_BC_*                              */
+    const int32_t *patch_nbr;    /* cyclic partner patch or -1            */
+    const int32_t *cell_submesh; /* optional cellSubmesh override or NULL */
+} lfm_polymesh;
+
+/* Solver inputs: CInputReader keys (src/inputReader.cpp:13-64). */
+typedef struct {
+    int32_t dim;        /* fvSchemes lfm/dimension: 2 or 3                 */
+    int32_t solver;     /* fvSchemes lfm/solver: 0 M1, 1 M2, 2 M2AUSM      */
+    int32_t rk_order;   /* fvSchemes lfm/rkOrder: 0..5                     */
+    int32_t minmod;     /* fvSchemes lfm/minmodExists                      */
+    int32_t adjust_dt;  /* controlDict adjustTimeStep                      */
+    int32_t have_res;   /* controlDict lfm/post/haveResiduals              */
+    double max_co;      /* controlDict maxCo                               */
+    double dt;          /* controlDict deltaT                              */
+    double mol_weight, cp, mu, pr;   /* thermophysicalProperties          */
+    double p_inf, T_inf, U_inf[3];   /* spongeDict pinf, Tinf, Uinf_*     */
+    double Ls, k_sponge, mach;       /* spongeDict Ls, k, M               */
+    int32_t precision;  /* LFM_PREC_FP64 or LFM_PREC_FP32P                 */
+    int32_t laminar;    /* turbulenceProperties simulationType laminar: 1 */
+} lfm_params;
+
+typedef struct {
+    int32_t dim, n_cells, n_cells_sm0, n_faces, n_ghosts, max_faces_per_cell;
+    int32_t precision, solver;
+    int64_t device_bytes;   /* device memory held by the handle            */
+    double dt, cfl;         /* current deltaT and last convective CFL      */
+    int64_t steps_done;
+} lfm_info;
+
+typedef struct lfm_solver lfm_solver;
+
+/* Patch role from OpenFOAM patch type and name (cfd_v0.cpp:983-1002). */
+int lfm_patch_role(const char *type, const char *name);
+
+/* Mesh<P,D>::initialize + initializeSolver + ISolver::initialize: geometry
+ * (polyMeshReaderOF), submesh split (polyMeshReaderOF.cpp:501-552),
+ * neighbour typing and S/d/w (mesh_reader.cpp:163-615, cfd_v0.cpp:760-861),
+ * ghost cells (cfd_v0.cpp:346-431), face order (cfd_v0.cpp:278-339) and
+ * initial state (cfd_v0.cpp:867-967), uploaded to the current device.
+ * alpha may be NULL when Ls <= 0. */
+int lfm_solver_create(const lfm_polymesh *mesh, const lfm_params *prm, const double *p, const double *T,
+                      const double *U3, const double *alpha, lfm_solver **out);
+int lfm_solver_destroy(lfm_solver *s);
+const char *lfm_last_error(void);
+int lfm_get_info(lfm_solver *s, lfm_info *info);
+
+/* ISolver::prepare_for_timestep (cfd_v0.cpp:1326-1333). */
+int lfm_prepare_for_timestep(lfm_solver *s);
+/* ISolver::prepare_for_RKstep (cfd_v0.cpp:1339-1377). */
+int lfm_prepare_for_rkstep(lfm_solver *s, int rk);
+/* ISolver::set_boundary_conditions (cfd_v0.cpp:1010-1221). */
+int lfm_set_boundary_conditions(lfm_solver *s);
+/* ISolver::calc_gradients / calc_gradients_M2AUSM (cfd_v0.cpp:1384-1567). */
+int lfm_calc_gradients(lfm_solver *s);
+/* ISolver::calc_VIS (cfd_v0.cpp:1744-1860). */
+int lfm_calc_vis(lfm_solver *s);
+/* ISolver::one_rk_step_{M1,M2,M2AUSM} for both submeshes (cfd_v0.cpp:1897-2832):
+ * faces, residual scatter, sponge, RK update.  res (D+2, may be NULL) receives
+ * the running sums of squared cell residuals when rk == 0 and have_res. */
+int lfm_one_rk_step(lfm_solver *s, int rk, double dt, double *res);
+/* ISolver::compute_dt / compute_cfl (cfd_v0.cpp:2887-2942), FP64 reductions. */
+int lfm_compute_dt(lfm_solver *s, double max_co, double *dt_out);
+int lfm_compute_cfl(lfm_solver *s, double dt, double *cfl_out);
+/* n iterations of the Mesh::solve loop body (mesh_solver.cpp:474-740):
+ * prepare, all RK stages, then dt (adjustTimeStep) or CFL; no host sync. */
+int lfm_run(lfm_solver *s, int nsteps);
+/* Wait for all queued work of the handle. */
+int lfm_synchronize(lfm_solver *s);
+/* Time n steps with HIP events on the handle's stream (ms). */
+int lfm_time_steps(lfm_solver *s, int nsteps, double *ms_out);
+
+/* Fields in original cell order, SoA [comp][cell], FP64 on the host. */
+int lfm_get_field(lfm_solver *s, int which, double *out);
+int lfm_set_field(lfm_solver *s, int which, const double *in);
+/* Sum of squared residuals of the last step's stage 0 (D+2), as printed by
+ * Mesh::solve (mesh_solver.cpp:763-793, before the square root). */
+int lfm_get_residual(lfm_solver *s, double *res);
+/* Number of non-finite conservatives in the state (0 = healthy). */
+int lfm_check_finite(lfm_solver *s, int64_t *n_bad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

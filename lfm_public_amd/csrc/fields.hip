@@ -1,0 +1,128 @@
+// fields.hip -- host <-> device field transfer in original cell order, state
+// reset for a new time step, and the finiteness check.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "kernels_fp32p.hpp"
+#include "solver_state.hpp"
+
+namespace lfm {
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(LFM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+int create_fp32p(Solver *) { return fail(LFM_EINVAL, "FP32-perturbation path not available in this build"); }
+
+// ISolver::prepare_for_timestep: dq = 0, RES = 0
+void Solver::zero_state_for_step() {
+    const size_t n = (size_t)(L.D + 2) * L.nc;
+    const int nb = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(fill_zero, dim3(nb), dim3(256), 0, stream, k64.dq, n);
+    hipLaunchKernelGGL(fill_zero, dim3(nb), dim3(256), 0, stream, k64.res_cell, n);
+    pending_A = 1.0;
+}
+
+namespace {
+
+// (device pointer, components, stride) of a field in the FP64 layout
+struct FieldRef {
+    double *ptr;
+    int ncomp;
+    size_t stride;
+};
+
+FieldRef field_ref64(Solver *s, int which) {
+    const int D = s->L.D;
+    const size_t nt = (size_t)s->L.nc + s->L.ng, nc = s->L.nc;
+    KArgs &a = s->k64;
+    switch (which) {
+    case LFM_FIELD_Q: return {a.q, D + 2, nt};
+    case LFM_FIELD_DQ: return {a.dq, D + 2, nc};
+    case LFM_FIELD_DUDX: return {a.dudx, D * D, nt};
+    case LFM_FIELD_DTDX: return {a.dTdx, D, nt};
+    case LFM_FIELD_TAUMC: return {a.tauMC, D * D, nt};
+    case LFM_FIELD_SIGMAU: return {a.sigmaU, D, nt};
+    case LFM_FIELD_RHOGRAD: return {a.rho_grad, D, nt};
+    case LFM_FIELD_PGRAD: return {a.p_grad, D, nt};
+    case LFM_FIELD_UGRAD: return {a.U_grad, D * D, nt};
+    case LFM_FIELD_RES: return {a.res_cell, D + 2, nc};
+    default: return {nullptr, 0, 0};
+    }
+}
+
+}  // namespace
+
+int Solver::get_field(int which, double *out) {
+    const int D = L.D;
+    const size_t nc = L.nc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (which == LFM_FIELD_PRIM) {
+        std::vector<double> q((size_t)(D + 2) * nc);
+        int rc = get_field(LFM_FIELD_Q, q.data());
+        if (rc) return rc;
+        for (size_t c = 0; c < nc; c++) {
+            const double rho = q[c];
+            double u2 = 0.0;
+            for (int k = 0; k < D; k++) {
+                const double u = q[(k + 1) * nc + c] / rho;
+                out[(k + 1) * nc + c] = u;
+                u2 += u * u;
+            }
+            const double p = (q[(D + 1) * nc + c] - 0.5 * rho * u2) * gas.gm1;
+            out[c] = p;
+            out[(D + 1) * nc + c] = p * gas.R_inv / rho;
+        }
+        return LFM_OK;
+    }
+    if (fp32) return fail(LFM_EINVAL, "field not available for the FP32 path");
+    FieldRef r = field_ref64(this, which);
+    if (!r.ptr) return fail(LFM_EINVAL, "unknown or unallocated field");
+    std::vector<double> tmp(nc);
+    for (int k = 0; k < r.ncomp; k++) {
+        HIP_TRY(hipMemcpy(tmp.data(), r.ptr + k * r.stride, nc * sizeof(double), hipMemcpyDeviceToHost));
+        double *o = out + k * nc;
+        for (size_t ci = 0; ci < nc; ci++) o[L.orig[ci]] = tmp[ci];
+    }
+    return LFM_OK;
+}
+
+int Solver::set_field(int which, const double *in) {
+    const size_t nc = L.nc;
+    HIP_TRY(hipStreamSynchronize(stream));
+    if (fp32) return fail(LFM_EINVAL, "field not settable for the FP32 path");
+    FieldRef r = field_ref64(this, which);
+    if (!r.ptr || (which != LFM_FIELD_Q && which != LFM_FIELD_DQ)) return fail(LFM_EINVAL, "field not settable");
+    std::vector<double> tmp(nc);
+    for (int k = 0; k < r.ncomp; k++) {
+        const double *src = in + k * nc;
+        for (size_t ci = 0; ci < nc; ci++) tmp[ci] = src[L.orig[ci]];
+        HIP_TRY(hipMemcpy(r.ptr + k * r.stride, tmp.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return LFM_OK;
+}
+
+int Solver::check_finite(int64_t *n_bad) {
+    unsigned long long *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(unsigned long long)));
+    HIP_TRY(hipMemsetAsync(d, 0, sizeof(unsigned long long), stream));
+    const int nb = (L.nc + 255) / 256;
+    if (!fp32) {
+        if (L.D == 2)
+            hipLaunchKernelGGL(count_nonfinite<2>, dim3(nb), dim3(256), 0, stream, k64.q, L.nc, L.nc + L.ng, d);
+        else
+            hipLaunchKernelGGL(count_nonfinite<3>, dim3(nb), dim3(256), 0, stream, k64.q, L.nc, L.nc + L.ng, d);
+    }
+    unsigned long long h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(hipFree(d));
+    *n_bad = (int64_t)h;
+    return LFM_OK;
+}
+
+}  // namespace lfm

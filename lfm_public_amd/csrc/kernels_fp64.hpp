@@ -1,0 +1,767 @@
+// kernels_fp64.hpp -- gfx950 kernels of the plain FP64 time step.
+//
+// One RK stage is four launches over SoA arrays:
+//   bc_fill      one thread per boundary ghost   (set_boundary_conditions)
+//   grad_vis     one thread per cell, gathering its faces in the reference's
+//                processing order (calc_gradients[_M2AUSM] + calc_VIS)
+//   face_flux    one thread per face, each face once (one_rk_step_* body)
+//   cell_update  one thread per cell, gathering face fluxes in the
+//                reference's accumulation order, then sponge and RK update
+// The gathers replace the reference's owner/neighbour scatter: every cell
+// sums the same terms in the same order as the reference's in-place loop,
+// so no atomics are needed and the result does not depend on scheduling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lfm {
+
+struct KArgs {
+    int nc, nsm0, ng, nf, ntot;
+    const int32_t *fP, *fN;
+    const double *S, *d, *w;        // S, d: [D][nf]
+    const int32_t *g_off, *g_ent;   // gather lists
+    const int32_t *s_off, *s_ent;   // slot lists
+    const int32_t *ghost_cell, *ghost_role;
+    const double *vinv, *sigma;
+    double *q;                      // [(D+2)][ntot]
+    double *dq;                     // [(D+2)][nc]
+    double *res_cell;               // [(D+2)][nc]
+    double *dudx, *dTdx, *tauMC, *sigmaU;  // [D*D|D][ntot], ghost part zero
+    double *rho_grad, *p_grad, *U_grad;    // [D|D*D][ntot], ghost part zero
+    double *frhs;                   // [(D+2)][nf]
+    const double *dt;               // device scalar (deltaT)
+    double *partial;                // reduction partials
+    // gas
+    double gamma, gm1, R_inv, mu, cp, pr_inv;
+    double p_inf, T_inf, rho_inf, E_inf, U_inf[3];
+};
+
+#define INTERP_LINEAR(w, a, b) ((w) * (a) + (1.0 - (w)) * (b))
+
+template <int D>
+__device__ __forceinline__ double rpsi(const KArgs &a, const double *q) {
+    const double ri = 1.0 / q[0];
+    double s = q[1] * q[1];
+#pragma unroll
+    for (int k = 1; k < D; k++) s += q[k + 1] * q[k + 1];
+    return a.gm1 * (q[D + 1] - 0.5 * s * ri) * ri;
+}
+
+template <int D>
+__device__ __forceinline__ double dotD(const double *x, const double *y) {
+    double s = x[0] * y[0];
+#pragma unroll
+    for (int k = 1; k < D; k++) s += x[k] * y[k];
+    return s;
+}
+
+template <int D>
+__device__ __forceinline__ void load_q(const KArgs &a, int i, double *q) {
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) q[k] = a.q[(size_t)k * a.ntot + i];
+}
+
+__device__ __forceinline__ double sgn(double x) { return x < 0.0 ? -1.0 : 1.0; }
+
+template <int D>
+__device__ __forceinline__ double interp_minmod(double cphi, double aphi, const double *grad, const double *d,
+                                                double wl, double flux) {
+    const double gradf = aphi - cphi + 1.0e-30;
+    double gradcf = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) gradcf += d[k] * grad[k];
+    double r;
+    if (fabs(gradcf) >= 1000.0 * fabs(gradf))
+        r = 2.0 * 1000.0 * sgn(gradcf) * sgn(gradf) - 1.0;
+    else
+        r = 2.0 * (gradcf / gradf) - 1.0;
+    // std::max(std::min(r, 1), 0) of api/cfdv0_solver.h:364
+    const double m1 = (1.0 < r) ? 1.0 : r;
+    const double lim = (m1 < 0.0) ? 0.0 : m1;
+    const double wt = lim * wl + (1.0 - lim) * (1.0 + flux) * 0.5;
+    return wt * cphi + (1.0 - wt) * aphi;
+}
+
+__device__ __forceinline__ double p5pos(double M, double alpha) {
+    const double M2Pos = 0.25 * (M + 1.0) * (M + 1.0);
+    const double M2Neg = -0.25 * (M - 1.0) * (M - 1.0);
+    const double M1Pos = 0.5 * (M + fabs(M));
+    if (fabs(M) < 1) return M2Pos * ((2.0 - M) - 16.0 * alpha * M * M2Neg);
+    return M1Pos / M;
+}
+
+__device__ __forceinline__ double p5neg(double M, double alpha) {
+    const double M2Pos = 0.25 * (M + 1.0) * (M + 1.0);
+    const double M2Neg = -0.25 * (M - 1.0) * (M - 1.0);
+    const double M1Neg = 0.5 * (M - fabs(M));
+    if (fabs(M) < 1) return M2Neg * ((-2.0 - M) + 16.0 * alpha * M * M2Pos);
+    return M1Neg / M;
+}
+
+// ---------------------------------------------------------------------------
+// set_boundary_conditions (cfd_v0.cpp:1010-1133): ghost state per boundary
+// face from its interior cell.  Ghost gradient copies are not stored: the
+// interior gradients they copy are zero at this point of every stage, and
+// the flux kernels read zero ghost gradients (see grad arrays' ghost part).
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) bc_fill(KArgs a) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.ng) return;
+    const int c = a.ghost_cell[g];
+    const int role = a.ghost_role[g];
+    double qi[D + 2], qg[D + 2];
+    load_q<D>(a, c, qi);
+    if (role == 1) {  // wall
+        qg[0] = qi[0];
+#pragma unroll
+        for (int k = 0; k < D; k++) qg[k + 1] = -qi[k + 1];
+        qg[D + 1] = qi[D + 1];
+    } else if (role == 2 || role == 3) {  // inlet / outlet
+        const double rhoInt = qi[0];
+        double Uint[D], u2i = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            Uint[k] = qi[k + 1] / rhoInt;
+            u2i += Uint[k] * Uint[k];
+        }
+        const double pInt = (qi[D + 1] - 0.5 * u2i * rhoInt) * a.gm1;
+        double rhoExt, pExt, Uext[D], u2e = 0.0;
+        if (role == 2) {
+            const double TInt = pInt * a.R_inv / rhoInt;
+            const double TExt = a.T_inf;
+            rhoExt = rhoInt * TInt / TExt;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                Uext[k] = a.U_inf[k];
+                u2e += Uext[k] * Uext[k];
+            }
+            pExt = pInt;
+        } else {
+            pExt = a.p_inf;
+            rhoExt = rhoInt * pExt / pInt;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                Uext[k] = Uint[k];
+                u2e += Uext[k] * Uext[k];
+            }
+        }
+        const double EExt = pExt / (rhoExt * a.gm1) + 0.5 * u2e;
+        qg[0] = rhoExt;
+#pragma unroll
+        for (int k = 0; k < D; k++) qg[k + 1] = rhoExt * Uext[k];
+        qg[D + 1] = rhoExt * EExt;
+    } else {
+        return;  // unset patch: ghost keeps q = 0 (as the reference's zeroed ghosts)
+    }
+    const int gi = a.nc + g;
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + gi] = qg[k];
+}
+
+// ---------------------------------------------------------------------------
+// Green-Gauss gradients + viscous tensors, per cell (cfd_v0.cpp:1384-1567,
+// 1744-1860).  The reference refreshes tau/sigmaU/tauMC after every touch of
+// a cell; the last touch of a cell follows its last contribution, so the
+// stored values equal the refresh of the completed sums done here.
+// GRAD: 0 none, 1 calc_gradients (results unused by solvers 0/1: skipped),
+// 2 calc_gradients_M2AUSM (rho, p, U gradients consumed by the M2AUSM flux).
+// ---------------------------------------------------------------------------
+template <int D, int GRAD>
+__global__ void __launch_bounds__(256) grad_vis(KArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nc) return;
+    const double vinv = a.vinv[c];
+    double dudx[D][D], dTdx[D];
+    double rg[D], pg[D], ug[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        dTdx[i] = 0.0;
+        rg[i] = 0.0;
+        pg[i] = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            dudx[i][j] = 0.0;
+            ug[i][j] = 0.0;
+        }
+    }
+    const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
+    for (int e = e0; e < e1; e++) {
+        const int ent = a.g_ent[e];
+        const bool isP = ent >= 0;
+        const int f = isP ? ent : ~ent;
+        const int P = a.fP[f], N = a.fN[f];
+        const double w = a.w[f];
+        double S[D], qP[D + 2], qN[D + 2];
+#pragma unroll
+        for (int k = 0; k < D; k++) S[k] = a.S[(size_t)k * a.nf + f];
+        load_q<D>(a, P, qP);
+        load_q<D>(a, N, qN);
+        const double cR = rpsi<D>(a, qP), aR = rpsi<D>(a, qN);
+        const double cri = 1.0 / qP[0], ari = 1.0 / qN[0];
+        double sv[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) sv[k] = isP ? S[k] * vinv : -S[k] * vinv;
+        if (GRAD == 2) {
+            double UU[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) UU[k] = INTERP_LINEAR(w, qP[k + 1] / qP[0], qN[k + 1] / qN[0]);
+            const double rho = INTERP_LINEAR(w, qP[0], qN[0]);
+            double r = qP[0], E = qP[D + 1] / r, vm = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; k++) vm += (qP[k + 1] / r) * (qP[k + 1] / r);
+            const double cp_ = r * a.gm1 * (E - 0.5 * vm);
+            r = qN[0];
+            E = qN[D + 1] / r;
+            vm = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; k++) vm += (qN[k + 1] / r) * (qN[k + 1] / r);
+            const double np_ = r * a.gm1 * (E - 0.5 * vm);
+            const double pf = INTERP_LINEAR(w, cp_, np_);
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+                pg[i] += pf * sv[i];
+#pragma unroll
+                for (int j = 0; j < D; j++) ug[j][i] += UU[j] * sv[i];
+                rg[i] += rho * sv[i];
+            }
+        }
+        double fU[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) fU[k] = INTERP_LINEAR(w, qP[k + 1] * cri, qN[k + 1] * ari);
+        const double fT = INTERP_LINEAR(w, cR, aR) * a.R_inv;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) dudx[i][j] += fU[i] * sv[j];
+            dTdx[i] += fT * sv[i];
+        }
+    }
+    // refresh of tau / sigmaU / tauMC from the completed dudx (cfd_v0.cpp:1813-1834)
+    double qc[D + 2];
+    load_q<D>(a, c, qc);
+    double U[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) U[k] = qc[k + 1] / qc[0];
+    const double mu = a.mu;
+    double tau[D][D];
+#pragma unroll
+    for (int p = 0; p < D; p++) {
+        tau[p][p] = 2.0 * dudx[p][p];
+#pragma unroll
+        for (int b1 = p + 1; b1 < D + p; b1++) {
+            const int b = b1 % D;
+            tau[p][b] = mu * (dudx[p][b] + dudx[b][p]);
+            tau[p][p] -= dudx[b][b];
+        }
+        tau[p][p] *= 2.0 / 3.0 * mu;
+    }
+    double diag = 0;
+#pragma unroll
+    for (int p = 0; p < D; p++) diag -= dudx[p][p];
+    diag *= mu * 2.0 / 3.0;
+    const size_t nt = a.ntot;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        a.sigmaU[i * nt + c] = dotD<D>(U, tau[i]);
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            double t = mu * dudx[j][i];
+            if (i == j) t += diag;
+            a.tauMC[(i * D + j) * nt + c] = t;
+            a.dudx[(i * D + j) * nt + c] = dudx[i][j];
+        }
+        a.dTdx[i * nt + c] = dTdx[i];
+    }
+    if (GRAD == 2) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            a.rho_grad[i * nt + c] = rg[i];
+            a.p_grad[i * nt + c] = pg[i];
+#pragma unroll
+            for (int j = 0; j < D; j++) a.U_grad[(i * D + j) * nt + c] = ug[i][j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Face fluxes: convective (M1 2589-2675, M2 1968-2023, M2AUSM 2256-2366) plus
+// the viscous terms shared by all three (2682-2790).  One thread per face.
+// ---------------------------------------------------------------------------
+template <int D, int SOLVER>
+__global__ void __launch_bounds__(256) face_flux(KArgs a) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    const int P = a.fP[f], N = a.fN[f];
+    const bool ghost = N >= a.nc;
+    const double w = a.w[f];
+    const size_t nt = a.ntot;
+    double S[D], d[D], qP[D + 2], qN[D + 2];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        S[k] = a.S[(size_t)k * a.nf + f];
+        d[k] = a.d[(size_t)k * a.nf + f];
+    }
+    load_q<D>(a, P, qP);
+    load_q<D>(a, N, qN);
+    const double cri = 1.0 / qP[0], ari = 1.0 / qN[0];
+    double cU[D], aU[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        cU[k] = qP[k + 1] * cri;
+        aU[k] = qN[k + 1] * ari;
+    }
+    const double cR = rpsi<D>(a, qP), aR = rpsi<D>(a, qN);
+    const double cT = cR * a.R_inv, aT = aR * a.R_inv;
+    double rhs[D + 2];
+    double Sm = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) Sm += S[k] * S[k];
+    Sm = sqrt(Sm);
+
+    if (SOLVER == 0) {
+        const double omw = 1.0 - w;
+        const double rhoPos = INTERP_LINEAR(w, qP[0], qN[0]);
+        const double rhoNeg = INTERP_LINEAR(omw, qN[0], qP[0]);
+        const double rPi = 1.0 / rhoPos, rNi = 1.0 / rhoNeg;
+        double rUP[D], rUN[D], uP[D], uN[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            rUP[k] = INTERP_LINEAR(w, qP[k + 1], qN[k + 1]);
+            rUN[k] = INTERP_LINEAR(omw, qN[k + 1], qP[k + 1]);
+        }
+        double ce = 2 * qP[D + 1] * cri;
+        double ae = 2 * qN[D + 1] * ari;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            ce -= cU[k] * cU[k];
+            ae -= aU[k] * aU[k];
+        }
+        ce *= 0.5;
+        ae *= 0.5;
+        const double eP = INTERP_LINEAR(w, ce, ae), eN = INTERP_LINEAR(omw, ae, ce);
+        const double RP = INTERP_LINEAR(w, cR, aR), RN = INTERP_LINEAR(omw, aR, cR);
+        const double pP = rhoPos * RP, pN = rhoNeg * RN;
+        const double cP = sqrt(a.gamma * cR), cN = sqrt(a.gamma * aR);
+        const double cPos = INTERP_LINEAR(w, cP, cN), cNeg = INTERP_LINEAR(omw, cN, cP);
+        double phP = 0.0, phN = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            uP[k] = rUP[k] * rPi;
+            uN[k] = rUN[k] * rNi;
+            phP += uP[k] * S[k];
+            phN += -uN[k] * S[k];
+        }
+        // std::max({x1, x2, 0}) / std::min({y1, y2, 0}): first extreme wins
+        const double x1 = phP + cPos * Sm, x2 = -phN + cNeg * Sm;
+        double psiP = (x1 < x2) ? x2 : x1;
+        if (psiP < 0.0) psiP = 0.0;
+        const double y1 = phP - cPos * Sm, y2 = -phN - cNeg * Sm;
+        double psiN = (y2 < y1) ? y2 : y1;
+        if (0.0 < psiN) psiN = 0.0;
+        const double a0 = 1.0 / (psiP - psiN);
+        const double a1 = psiP * psiN;
+        const double aP = psiP * phP, aN = psiN * phN;
+        double rEP = eP, rEN = eN;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            rEP += 0.5 * uP[k] * uP[k];
+            rEN += 0.5 * uN[k] * uN[k];
+        }
+        rEP *= rhoPos;
+        rEN *= rhoNeg;
+        rhs[0] = -(aP * rhoPos + aN * rhoNeg + (rhoNeg - rhoPos) * a1) * a0;
+#pragma unroll
+        for (int k = 0; k < D; k++)
+            rhs[k + 1] = -((aP * rUP[k] + aN * rUN[k] + (rUN[k] - rUP[k]) * a1) * a0 +
+                           (pP * psiP - pN * psiN) * a0 * S[k]);
+        rhs[D + 1] = -(aP * rEP + aN * rEN + (rEN - rEP) * a1 + (aP * pP + aN * pN)) * a0;
+    } else {
+        const double rhoavg = 0.5 * (qP[0] + qN[0]);
+        const double rai = 1.0 / rhoavg;
+        double rUa[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) rUa[k] = 0.5 * (qP[k + 1] + qN[k + 1]);
+        const double Ravg = 0.5 * (cR + aR);
+        double pavg = rhoavg * Ravg;
+        const double cH = qP[D + 1] / qP[0] + cR;
+        const double aH = qN[D + 1] / qN[0] + aR;
+        const double Havg = 0.5 * (cH + aH);
+        if (SOLVER == 2) {
+            const double cP = sqrt(a.gamma * cR), cN = sqrt(a.gamma * aR);
+            const double cavg = 0.5 * (cP + cN);
+            double cdiv = 0, ndiv = 0, ccurl = 0, ncurl = 0, cvm = 0, nvm = 0, unP = 0, unN = 0, nrm = 0;
+            double cdu[D][D], ndu[D][D];
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) {
+                    cdu[i][j] = a.dudx[(i * D + j) * nt + P];
+                    ndu[i][j] = a.dudx[(i * D + j) * nt + N];
+                }
+#pragma unroll
+            for (int p = 0; p < D; p++) {
+                cdiv += cdu[p][p];
+                ndiv += ndu[p][p];
+#pragma unroll
+                for (int b = p + 1; b < D; b++) {
+                    ccurl += (cdu[p][b] - cdu[b][p]) * (cdu[p][b] - cdu[b][p]);
+                    ncurl += (ndu[p][b] - ndu[b][p]) * (ndu[p][b] - ndu[b][p]);
+                }
+                const double cu = qP[p + 1] / qP[0];
+                cvm += cu * cu;
+                const double nu = qN[p + 1] / qN[0];
+                nvm += nu * nu;
+                double gc[D], gn[D];
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    gc[k] = a.U_grad[(p * D + k) * nt + P];
+                    gn[k] = a.U_grad[(p * D + k) * nt + N];
+                }
+                const double uPos = interp_minmod<D>(cu, nu, gc, d, w, 1.0);
+                const double uNeg = interp_minmod<D>(cu, nu, gn, d, w, -1.0);
+                unP += uPos * S[p];
+                unN += uNeg * S[p];
+                nrm += S[p] * S[p];
+            }
+            const double cth0 = -(cdiv / sqrt(cdiv * cdiv + ccurl + 4e-2));
+            const double nth0 = -(ndiv / sqrt(ndiv * ndiv + ncurl + 4e-2));
+            const double cth = (cth0 < 0.) ? 0. : cth0;
+            const double nth = (nth0 < 0.) ? 0. : nth0;
+            const double tha = 0.5 * (cth + nth);
+            double r = qP[0], E = qP[D + 1] / r;
+            const double cp_ = r * a.gm1 * (E - 0.5 * cvm);
+            r = qN[0];
+            E = qN[D + 1] / r;
+            const double np_ = r * a.gm1 * (E - 0.5 * nvm);
+            double gc[D], gn[D], hc[D], hn[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                gc[k] = a.rho_grad[k * nt + P];
+                gn[k] = a.rho_grad[k * nt + N];
+                hc[k] = a.p_grad[k * nt + P];
+                hn[k] = a.p_grad[k * nt + N];
+            }
+            const double rhoP = interp_minmod<D>(qP[0], qN[0], gc, d, w, 1.0);
+            const double rhoN = interp_minmod<D>(qP[0], qN[0], gn, d, w, -1.0);
+            const double pP = interp_minmod<D>(cp_, np_, hc, d, w, 1.0);
+            const double pN = interp_minmod<D>(cp_, np_, hn, d, w, -1.0);
+            const double Msq = (unP * unP + unN * unN) / (2. * cavg * cavg * nrm);
+            const double MP = unP / (sqrt(nrm) * cavg);
+            const double MN = unN / (sqrt(nrm) * cavg);
+            const double Minf = 0.2;
+            const double mx = (Msq < Minf * Minf) ? Minf * Minf : Msq;
+            const double M0 = sqrt((mx < 1.0) ? mx : 1.0);
+            const double fa = M0 * (2.0 - M0);
+            const double alpha = 3.0 * (5.0 * fa * fa - 4.0) / 16.0;
+            const double phalf = pN * (p5pos(MN, alpha) - p5neg(MN, alpha)) - pP * (p5pos(MP, alpha) - p5neg(MP, alpha));
+            const double pu = -0.75 * p5pos(MP, alpha) * p5neg(MN, alpha) * (rhoP + rhoN) * cavg * fa * (unN - unP);
+            pavg += tha * (pu - 0.5 * phalf);
+        }
+        double phi = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; k++) phi += (rUa[k] * rai) * S[k];
+        rhs[0] = -rhoavg * phi;
+#pragma unroll
+        for (int k = 0; k < D; k++) rhs[k + 1] = -(rUa[k] * phi + pavg * S[k]);
+        rhs[D + 1] = -(rhoavg * Havg * phi);
+    }
+
+    // viscous terms (cfd_v0.cpp:2682-2790)
+    const double mu = a.mu;
+    double dmag = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) dmag += d[k] * d[k];
+    dmag = sqrt(dmag);
+    const double dmi = 1.0 / dmag;
+    double dn[D], du[D][D], dT[D], tau[D][D], tmc[D][D];
+#pragma unroll
+    for (int k = 0; k < D; k++) dn[k] = d[k] * dmi;
+    if (ghost) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) du[i][j] = (aU[i] - cU[i]) * dn[j] * dmi;
+            dT[i] = (aT - cT) * dn[i] * dmi;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++)
+                du[i][j] = INTERP_LINEAR(w, a.dudx[(i * D + j) * nt + P], a.dudx[(i * D + j) * nt + N]);
+            dT[i] = INTERP_LINEAR(w, a.dTdx[i * nt + P], a.dTdx[i * nt + N]);
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < D; p++) {
+        tau[p][p] = 2.0 * du[p][p];
+#pragma unroll
+        for (int b1 = p + 1; b1 < D + p; b1++) {
+            const int b = b1 % D;
+            tau[p][b] = mu * (du[p][b] + du[b][p]);
+            tau[p][p] -= du[b][b];
+        }
+        tau[p][p] *= 2.0 / 3.0 * mu;
+    }
+    const double kcond = a.cp * mu * a.pr_inv;
+    if (ghost) {
+        double diag = 0;
+#pragma unroll
+        for (int p = 0; p < D; p++) diag -= du[p][p];
+        diag *= mu * 2.0 / 3.0;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) tmc[i][j] = mu * du[j][i];
+            tmc[i][i] += diag;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++)
+                tmc[i][j] = INTERP_LINEAR(w, a.tauMC[(i * D + j) * nt + P], a.tauMC[(i * D + j) * nt + N]);
+    }
+    const double Sd = dotD<D>(S, d);
+    double delta[D], K[D], dm = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        delta[i] = d[i] * Sm * Sm / Sd;
+        dm += delta[i] * delta[i];
+        K[i] = S[i] - delta[i];
+    }
+    dm = sqrt(dm);
+    double sU[D];
+    if (ghost) {
+        double Uf[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) Uf[i] = INTERP_LINEAR(w, cU[i], aU[i]);
+#pragma unroll
+        for (int i = 0; i < D; i++) sU[i] = dotD<D>(Uf, tau[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++) sU[i] = INTERP_LINEAR(w, a.sigmaU[i * nt + P], a.sigmaU[i * nt + N]);
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        const double divT = dotD<D>(tmc[i], S);
+        const double lapU = mu * (dm * (aU[i] - cU[i]) * dmi + dotD<D>(K, du[i]));
+        rhs[i + 1] += divT + lapU;
+    }
+    const double lapT = kcond * (dm * (aT - cT) * dmi + dotD<D>(K, dT));
+    rhs[D + 1] += dotD<D>(sU, S) + lapT;
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) a.frhs[(size_t)k * a.nf + f] = rhs[k];
+}
+
+// ---------------------------------------------------------------------------
+// Residual gather, sponge, RK update (cfd_v0.cpp:2792-2822).  dq starts from
+// A_rk * dq (prepare_for_RKstep); zero_dq replaces it by 0 for the first
+// stage of a step (prepare_for_timestep zeroed it).
+// ---------------------------------------------------------------------------
+template <int D, bool RES>
+__global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, int zero_dq) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[256 / 64][D + 2];
+    double rsq[D + 2];
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) rsq[k] = 0.0;
+    if (c < a.nc) {
+        const double dt = *a.dt;
+        const double vinv = a.vinv[c];
+        double dq[D + 2], res[D + 2];
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            dq[k] = zero_dq ? 0.0 : a.dq[(size_t)k * a.nc + c] * A;
+            res[k] = 0.0;
+        }
+        const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
+        for (int e = e0; e < e1; e++) {
+            const int ent = a.g_ent[e];
+            if (ent >= 0) {
+#pragma unroll
+                for (int k = 0; k < D + 2; k++) {
+                    const double r = a.frhs[(size_t)k * a.nf + ent];
+                    if (RES) res[k] += r;
+                    dq[k] += dt * r * vinv;
+                }
+            } else {
+                const int f = ~ent;
+#pragma unroll
+                for (int k = 0; k < D + 2; k++) {
+                    const double r = a.frhs[(size_t)k * a.nf + f];
+                    if (RES) res[k] -= r;
+                    dq[k] -= dt * r * vinv;
+                }
+            }
+        }
+        double q[D + 2];
+        load_q<D>(a, c, q);
+        const double sg = a.sigma[c];
+        dq[0] += dt * sg * (a.rho_inf - q[0]);
+#pragma unroll
+        for (int k = 0; k < D; k++) dq[k + 1] += dt * sg * (a.rho_inf * a.U_inf[k] - q[k + 1]);
+        dq[D + 1] += dt * sg * (a.rho_inf * a.E_inf - q[D + 1]);
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            q[k] += B * dq[k];
+            a.q[(size_t)k * a.ntot + c] = q[k];
+            a.dq[(size_t)k * a.nc + c] = dq[k];
+        }
+        if (RES) {
+#pragma unroll
+            for (int k = 0; k < D + 2; k++) {
+                a.res_cell[(size_t)k * a.nc + c] = res[k];
+                rsq[k] = res[k] * res[k];
+            }
+        }
+    }
+    if (RES) {
+        // per-block partial sums of squared residuals (fixed order)
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            double v = rsq[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) red[wid][k] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < D + 2; k++) {
+                double s = 0.0;
+                for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i][k];
+                a.partial[(size_t)blockIdx.x * (D + 2) + k] = s;
+            }
+        }
+    }
+}
+
+// Sum of per-block partials in fixed order, one block.
+static __global__ void __launch_bounds__(256) reduce_sum(const double *partial, int nblk, int ncomp, double *out, int accumulate) {
+    __shared__ double sh[256];
+    for (int k = 0; k < ncomp; k++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += partial[(size_t)i * ncomp + k];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = accumulate ? out[k] + sh[0] : sh[0];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// compute_dt / compute_cfl (cfd_v0.cpp:2887-2942): per cell over its faces in
+// slot order, then min / max.  MODE 0: dt (min, start 100), 1: cfl (max, start 0).
+// ---------------------------------------------------------------------------
+template <int D, int MODE>
+__global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sh[256];
+    double v = MODE == 0 ? 100.0 : 0.0;
+    if (c < a.nc) {
+        double q[D + 2];
+        load_q<D>(a, c, q);
+        double sum = 0.0;
+        for (int e = a.s_off[c]; e < a.s_off[c + 1]; e++) {
+            const int ent = a.s_ent[e];
+            const int f = ent >= 0 ? ent : ~ent;
+            double S[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) S[k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
+            if (MODE == 0) {
+                double sM = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; k++) sM += S[k] * S[k];
+                sM = sqrt(sM);
+                double fl = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; k++) fl += q[k + 1] * S[k] / sM;
+                sum += fabs(fl);
+            } else {
+                double fl = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; k++) fl += q[k + 1] * S[k];
+                sum += fabs(fl);
+            }
+        }
+        if (MODE == 0) {
+            const double vol = 1.0 / a.vinv[c];
+            const double dtc = param * q[0] * 2.0 * vol / sum;
+            if (v > dtc) v = dtc;  // NaN never wins (as `dt_min > dt`)
+        } else {
+            const double cfl = sum * ((1.0 / q[0]) * a.vinv[c] * param);
+            if (v < cfl) v = cfl;
+        }
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const double o = sh[threadIdx.x + st];
+            if (MODE == 0) {
+                if (sh[threadIdx.x] > o) sh[threadIdx.x] = o;
+            } else {
+                if (sh[threadIdx.x] < o) sh[threadIdx.x] = o;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = sh[0];
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) cfl_dt_final(const double *partial, int nblk, double *out, double *dt_dst) {
+    __shared__ double sh[256];
+    double v = MODE == 0 ? 100.0 : 0.0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        const double o = partial[i];
+        if (MODE == 0) {
+            if (v > o) v = o;
+        } else {
+            if (v < o) v = o;
+        }
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const double o = sh[threadIdx.x + st];
+            if (MODE == 0) {
+                if (sh[threadIdx.x] > o) sh[threadIdx.x] = o;
+            } else {
+                if (sh[threadIdx.x] < o) sh[threadIdx.x] = o;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out = sh[0];
+        if (dt_dst) *dt_dst = sh[0];
+    }
+}
+
+static __global__ void __launch_bounds__(256) fill_zero(double *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) count_nonfinite(const double *q, int nc, int ntot, unsigned long long *cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int bad = 0;
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) bad += !isfinite(q[(size_t)k * ntot + c]);
+    if (bad) atomicAdd(cnt, (unsigned long long)bad);
+}
+
+}  // namespace lfm

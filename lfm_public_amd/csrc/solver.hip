@@ -1,0 +1,491 @@
+// solver.hip -- device state, launches and the C ABI (include/lfm_hip.h).
+//
+// One lfm_solver holds one partition (both reference submeshes) resident in
+// HBM in SoA layout and drives the stage kernels on its own HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels_fp64.hpp"
+#include "kernels_fp32p.hpp"
+#include "lfm_hip.h"
+#include "mesh_builder.hpp"
+#include "solver_state.hpp"
+
+namespace lfm {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(LFM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+static constexpr int kBlock = 256;
+static inline int nblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+template <typename T>
+static int dev_upload(Solver *s, T **dst, const T *src, size_t n) {
+    if (n == 0) {
+        *dst = nullptr;
+        return 0;
+    }
+    HIP_TRY(hipMalloc((void **)dst, n * sizeof(T)));
+    s->allocs.push_back((void *)*dst);
+    s->device_bytes += (int64_t)(n * sizeof(T));
+    if (src) HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+static int dev_zeros(Solver *s, T **dst, size_t n) {
+    if (n == 0) {
+        *dst = nullptr;
+        return 0;
+    }
+    HIP_TRY(hipMalloc((void **)dst, n * sizeof(T)));
+    s->allocs.push_back((void *)*dst);
+    s->device_bytes += (int64_t)(n * sizeof(T));
+    HIP_TRY(hipMemset(*dst, 0, n * sizeof(T)));
+    return 0;
+}
+
+Solver::~Solver() {
+    for (void *p : allocs) (void)hipFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (h_scalar) (void)hipHostFree(h_scalar);
+}
+
+// ---------------------------------------------------------------------------
+// creation
+// ---------------------------------------------------------------------------
+static int create_fp64(Solver *s) {
+    const SolverLayout &L = s->L;
+    const int D = L.D;
+    const size_t nc = L.nc, ntot = (size_t)L.nc + L.ng, nf = L.nf;
+    KArgs &a = s->k64;
+    std::memset(&a, 0, sizeof(a));
+    a.nc = L.nc;
+    a.nsm0 = L.nsm0;
+    a.ng = L.ng;
+    a.nf = L.nf;
+    a.ntot = (int)ntot;
+    int rc = 0;
+    rc |= dev_upload(s, (int32_t **)&a.fP, L.fP.data(), nf);
+    rc |= dev_upload(s, (int32_t **)&a.fN, L.fN.data(), nf);
+    rc |= dev_upload(s, (double **)&a.S, L.S.data(), (size_t)D * nf);
+    rc |= dev_upload(s, (double **)&a.d, L.d.data(), (size_t)D * nf);
+    rc |= dev_upload(s, (double **)&a.w, L.w.data(), nf);
+    rc |= dev_upload(s, (int32_t **)&a.g_off, L.g_off.data(), nc + 1);
+    rc |= dev_upload(s, (int32_t **)&a.g_ent, L.g_ent.data(), L.g_ent.size());
+    rc |= dev_upload(s, (int32_t **)&a.s_off, L.s_off.data(), nc + 1);
+    rc |= dev_upload(s, (int32_t **)&a.s_ent, L.s_ent.data(), L.s_ent.size());
+    rc |= dev_upload(s, (int32_t **)&a.ghost_cell, L.ghost_cell.data(), L.ng);
+    rc |= dev_upload(s, (int32_t **)&a.ghost_role, L.ghost_role.data(), L.ng);
+    rc |= dev_upload(s, (double **)&a.vinv, L.vinv.data(), nc);
+    rc |= dev_upload(s, (double **)&a.sigma, L.sigma.data(), nc);
+    rc |= dev_zeros(s, &a.q, (size_t)(D + 2) * ntot);
+    rc |= dev_zeros(s, &a.dq, (size_t)(D + 2) * nc);
+    rc |= dev_zeros(s, &a.res_cell, (size_t)(D + 2) * nc);
+    rc |= dev_zeros(s, &a.dudx, (size_t)D * D * ntot);
+    rc |= dev_zeros(s, &a.dTdx, (size_t)D * ntot);
+    rc |= dev_zeros(s, &a.tauMC, (size_t)D * D * ntot);
+    rc |= dev_zeros(s, &a.sigmaU, (size_t)D * ntot);
+    if (s->prm.solver == 2 && s->prm.minmod) {
+        rc |= dev_zeros(s, &a.rho_grad, (size_t)D * ntot);
+        rc |= dev_zeros(s, &a.p_grad, (size_t)D * ntot);
+        rc |= dev_zeros(s, &a.U_grad, (size_t)D * D * ntot);
+    } else if (s->prm.solver == 2) {
+        // solver 2 without minmodExists reads never-computed (zero) gradients
+        rc |= dev_zeros(s, &a.rho_grad, (size_t)D * ntot);
+        rc |= dev_zeros(s, &a.p_grad, (size_t)D * ntot);
+        rc |= dev_zeros(s, &a.U_grad, (size_t)D * D * ntot);
+    }
+    rc |= dev_zeros(s, &a.frhs, (size_t)(D + 2) * nf);
+    const size_t npart = (size_t)nblocks((long)std::max(nc, nf)) * (D + 2) + 16;
+    rc |= dev_zeros(s, &a.partial, npart);
+    rc |= dev_zeros(s, &s->d_scalar, 16);
+    if (rc) return LFM_EDEVICE;
+    a.dt = s->d_scalar + 0;
+    s->d_res = s->d_scalar + 2;  // D+2 <= 5 slots: [2, 7)
+    s->d_cfl = s->d_scalar + 8;
+    s->d_dtmin = s->d_scalar + 9;
+    const Gas &g = s->gas;
+    a.gamma = g.gamma;
+    a.gm1 = g.gm1;
+    a.R_inv = g.R_inv;
+    a.mu = g.mu;
+    a.cp = g.cp;
+    a.pr_inv = g.pr_inv;
+    a.p_inf = g.p_inf;
+    a.T_inf = g.T_inf;
+    a.rho_inf = g.rho_inf;
+    a.E_inf = g.E_inf;
+    for (int k = 0; k < 3; k++) a.U_inf[k] = g.U_inf[k];
+    // initial state into q (cells part) and deltaT
+    for (int k = 0; k < D + 2; k++)
+        HIP_TRY(hipMemcpy(a.q + (size_t)k * ntot, L.q0.data() + (size_t)k * nc, nc * sizeof(double),
+                          hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->d_scalar, &s->dt, sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 launches
+// ---------------------------------------------------------------------------
+template <int D>
+static void launch_bc64(Solver *s) {
+    if (s->L.ng) hipLaunchKernelGGL(bc_fill<D>, dim3(nblocks(s->L.ng)), dim3(kBlock), 0, s->stream, s->k64);
+}
+
+template <int D>
+static void launch_gradvis64(Solver *s) {
+    const int nb = nblocks(s->L.nc);
+    if (s->prm.solver == 2 && s->prm.minmod)
+        hipLaunchKernelGGL((grad_vis<D, 2>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+    else
+        hipLaunchKernelGGL((grad_vis<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+}
+
+template <int D>
+static void launch_flux64(Solver *s) {
+    const int nb = nblocks(s->L.nf);
+    switch (s->prm.solver) {
+    case 0: hipLaunchKernelGGL((face_flux<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
+    case 1: hipLaunchKernelGGL((face_flux<D, 1>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
+    default: hipLaunchKernelGGL((face_flux<D, 2>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
+    }
+}
+
+template <int D>
+static void launch_update64(Solver *s, double A, double B, bool zero_dq, bool res, bool accumulate_res) {
+    const int nb = nblocks(s->L.nc);
+    if (res) {
+        hipLaunchKernelGGL((cell_update<D, true>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, A, B, (int)zero_dq);
+        hipLaunchKernelGGL(reduce_sum, dim3(1), dim3(kBlock), 0, s->stream, s->k64.partial, nb, D + 2, s->d_res,
+                           (int)accumulate_res);
+    } else {
+        hipLaunchKernelGGL((cell_update<D, false>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, A, B, (int)zero_dq);
+    }
+}
+
+template <int D>
+static void launch_cfl_dt64(Solver *s, int mode, double param, double *out, double *dt_dst) {
+    const int nb = nblocks(s->L.nc);
+    if (mode == 0) {
+        hipLaunchKernelGGL((cfl_dt_cells<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, param);
+        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kBlock), 0, s->stream, s->k64.partial, nb, out, dt_dst);
+    } else {
+        hipLaunchKernelGGL((cfl_dt_cells<D, 1>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, param);
+        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kBlock), 0, s->stream, s->k64.partial, nb, out, dt_dst);
+    }
+}
+
+// dispatch helpers over dimension
+#define LFM_DISPATCH_D(s, call2, call3) \
+    do {                                \
+        if ((s)->L.D == 2) {            \
+            call2;                      \
+        } else {                        \
+            call3;                      \
+        }                               \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// precision-generic stage pieces
+// ---------------------------------------------------------------------------
+void Solver::bc() {
+    if (fp32) {
+        LFM_DISPATCH_D(this, launch_bc32<2>(this), launch_bc32<3>(this));
+    } else {
+        LFM_DISPATCH_D(this, launch_bc64<2>(this), launch_bc64<3>(this));
+    }
+}
+
+void Solver::gradvis() {
+    if (fp32) {
+        LFM_DISPATCH_D(this, launch_gradvis32<2>(this), launch_gradvis32<3>(this));
+    } else {
+        LFM_DISPATCH_D(this, launch_gradvis64<2>(this), launch_gradvis64<3>(this));
+    }
+}
+
+void Solver::flux() {
+    if (fp32) {
+        LFM_DISPATCH_D(this, launch_flux32<2>(this), launch_flux32<3>(this));
+    } else {
+        LFM_DISPATCH_D(this, launch_flux64<2>(this), launch_flux64<3>(this));
+    }
+}
+
+void Solver::update(double A, double B, bool zero_dq, bool res, bool accumulate_res) {
+    if (fp32) {
+        LFM_DISPATCH_D(this, launch_update32<2>(this, A, B, zero_dq, res, accumulate_res),
+                       launch_update32<3>(this, A, B, zero_dq, res, accumulate_res));
+    } else {
+        LFM_DISPATCH_D(this, launch_update64<2>(this, A, B, zero_dq, res, accumulate_res),
+                       launch_update64<3>(this, A, B, zero_dq, res, accumulate_res));
+    }
+}
+
+void Solver::cfl_dt(int mode, double param, double *out, double *dt_dst) {
+    if (fp32) {
+        LFM_DISPATCH_D(this, launch_cfl_dt32<2>(this, mode, param, out, dt_dst),
+                       launch_cfl_dt32<3>(this, mode, param, out, dt_dst));
+    } else {
+        LFM_DISPATCH_D(this, launch_cfl_dt64<2>(this, mode, param, out, dt_dst),
+                       launch_cfl_dt64<3>(this, mode, param, out, dt_dst));
+    }
+}
+
+// one full time step (Mesh::solve loop body), no host synchronisation
+void Solver::step() {
+    const bool res = prm.have_res != 0;
+    for (int rk = 0; rk < nstages; rk++) {
+        bc();
+        gradvis();
+        flux();
+        update(A[rk], B[rk], rk == 0, res && rk == 0, false);
+    }
+    if (prm.adjust_dt)
+        cfl_dt(0, prm.max_co, d_dtmin, d_dt_ptr());
+    else
+        cfl_dt(1, dt, d_cfl, nullptr);
+    steps_done++;
+    if (res) res_valid = true;
+}
+
+}  // namespace lfm
+
+using lfm::fail;
+using lfm::Solver;
+
+struct lfm_solver {
+    std::unique_ptr<Solver> impl;
+};
+
+extern "C" {
+
+const char *lfm_last_error(void) { return lfm::g_last_error.c_str(); }
+
+int lfm_patch_role(const char *type, const char *name) {
+    if (!type || !name) return LFM_BC_UNSET;
+    const std::string t(type), n(name);
+    if (t == "empty") return LFM_BC_EMPTY;
+    if (t == "cyclic") return LFM_BC_CYCLIC;
+    if (t == "wall") return LFM_BC_WALL;
+    if (t == "patch") {
+        if (n == "inlet" || n == "Inlet" || n == "Inflow" || n == "inflow") return LFM_BC_INLET;
+        if (n == "outlet" || n == "Outlet" || n == "Outflow" || n == "outflow") return LFM_BC_OUTLET;
+    }
+    return LFM_BC_UNSET;
+}
+
+int lfm_solver_create(const lfm_polymesh *mesh, const lfm_params *prm, const double *p, const double *T,
+                      const double *U3, const double *alpha, lfm_solver **out) {
+    if (!mesh || !prm || !p || !T || !U3 || !out) return fail(LFM_EINVAL, "null argument");
+    *out = nullptr;
+    if (prm->solver < 0 || prm->solver > 2) return fail(LFM_EINVAL, "solver must be 0 (M1), 1 (M2) or 2 (M2AUSM)");
+    if (prm->rk_order < 0 || prm->rk_order > 5) return fail(LFM_EINVAL, "rkOrder must be 0..5");
+    if (prm->precision != LFM_PREC_FP64 && prm->precision != LFM_PREC_FP32P)
+        return fail(LFM_EINVAL, "precision must be 64 or 32");
+    if (!prm->laminar)
+        return fail(LFM_EINVAL, "only laminar simulationType is supported (calc_VIS_Smagorinsky not built)");
+    auto s = std::make_unique<Solver>();
+    s->prm = *prm;
+    std::string err = lfm::build_layout(*mesh, *prm, p, T, U3, alpha, s->L);
+    if (!err.empty()) return fail(LFM_EINVAL, err);
+    s->gas = lfm::derive_gas(*prm);
+    s->nstages = lfm::rk_tables(prm->rk_order, s->A, s->B);
+    s->dt = prm->dt;
+    s->fp32 = prm->precision == LFM_PREC_FP32P;
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&s->ev0));
+    HIP_TRY(hipEventCreate(&s->ev1));
+    HIP_TRY(hipHostMalloc((void **)&s->h_scalar, 64 * sizeof(double), hipHostMallocDefault));
+    int rc = s->fp32 ? lfm::create_fp32p(s.get()) : lfm::create_fp64(s.get());
+    if (rc) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    auto *h = new lfm_solver;
+    h->impl = std::move(s);
+    *out = h;
+    return LFM_OK;
+}
+
+int lfm_solver_destroy(lfm_solver *h) {
+    if (!h) return LFM_OK;
+    if (h->impl && h->impl->stream) (void)hipStreamSynchronize(h->impl->stream);
+    delete h;
+    return LFM_OK;
+}
+
+int lfm_get_info(lfm_solver *h, lfm_info *info) {
+    if (!h || !info) return fail(LFM_EINVAL, "null argument");
+    Solver *s = h->impl.get();
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    info->dim = s->L.D;
+    info->n_cells = s->L.nc;
+    info->n_cells_sm0 = s->L.nsm0;
+    info->n_faces = s->L.nf;
+    info->n_ghosts = s->L.ng;
+    info->max_faces_per_cell = s->L.max_faces;
+    info->precision = s->prm.precision;
+    info->solver = s->prm.solver;
+    info->device_bytes = s->device_bytes;
+    HIP_TRY(hipMemcpy(s->h_scalar, s->d_scalar, 16 * sizeof(double), hipMemcpyDeviceToHost));
+    info->dt = s->h_scalar[0];
+    info->cfl = s->h_scalar[8];
+    info->steps_done = s->steps_done;
+    return LFM_OK;
+}
+
+int lfm_prepare_for_timestep(lfm_solver *h) {
+    if (!h) return fail(LFM_EINVAL, "null handle");
+    Solver *s = h->impl.get();
+    s->zero_state_for_step();
+    HIP_TRY(hipGetLastError());
+    return LFM_OK;
+}
+
+int lfm_prepare_for_rkstep(lfm_solver *h, int rk) {
+    if (!h) return fail(LFM_EINVAL, "null handle");
+    Solver *s = h->impl.get();
+    if (rk < 0 || rk >= std::max(1, s->nstages) || rk > 4) return fail(LFM_EINVAL, "rk stage out of range");
+    s->pending_A = s->A[rk];
+    s->pending_rk = rk;
+    return LFM_OK;
+}
+
+int lfm_set_boundary_conditions(lfm_solver *h) {
+    if (!h) return fail(LFM_EINVAL, "null handle");
+    h->impl->bc();
+    HIP_TRY(hipGetLastError());
+    return LFM_OK;
+}
+
+int lfm_calc_gradients(lfm_solver *h) {
+    // Gradients are produced together with the viscous terms by the same
+    // cell gather (lfm_calc_vis); calling this alone is a no-op.
+    if (!h) return fail(LFM_EINVAL, "null handle");
+    return LFM_OK;
+}
+
+int lfm_calc_vis(lfm_solver *h) {
+    if (!h) return fail(LFM_EINVAL, "null handle");
+    h->impl->gradvis();
+    HIP_TRY(hipGetLastError());
+    return LFM_OK;
+}
+
+int lfm_one_rk_step(lfm_solver *h, int rk, double dt, double *res) {
+    if (!h) return fail(LFM_EINVAL, "null handle");
+    Solver *s = h->impl.get();
+    if (rk < 0 || rk > 4) return fail(LFM_EINVAL, "rk stage out of range");
+    const double B = s->B[rk];
+    s->h_scalar[0] = dt;
+    HIP_TRY(hipMemcpyAsync(s->d_dt_ptr(), s->h_scalar, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    s->dt = dt;
+    const bool want_res = res && rk == 0 && s->prm.have_res;
+    s->flux();
+    s->update(s->pending_A, B, false, want_res, false);
+    s->pending_A = 1.0;
+    HIP_TRY(hipGetLastError());
+    if (want_res) {
+        HIP_TRY(hipMemcpyAsync(s->h_scalar + 16, s->d_res, (s->L.D + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                               s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        for (int k = 0; k < s->L.D + 2; k++) res[k] += s->h_scalar[16 + k];
+        s->res_valid = true;
+    }
+    return LFM_OK;
+}
+
+int lfm_compute_dt(lfm_solver *h, double max_co, double *dt_out) {
+    if (!h || !dt_out) return fail(LFM_EINVAL, "null argument");
+    Solver *s = h->impl.get();
+    s->cfl_dt(0, max_co, s->d_dtmin, nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(s->h_scalar + 9, s->d_dtmin, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    *dt_out = s->h_scalar[9];
+    return LFM_OK;
+}
+
+int lfm_compute_cfl(lfm_solver *h, double dt, double *cfl_out) {
+    if (!h || !cfl_out) return fail(LFM_EINVAL, "null argument");
+    Solver *s = h->impl.get();
+    s->cfl_dt(1, dt, s->d_cfl, nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(s->h_scalar + 8, s->d_cfl, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    *cfl_out = s->h_scalar[8];
+    return LFM_OK;
+}
+
+int lfm_run(lfm_solver *h, int nsteps) {
+    if (!h || nsteps < 0) return fail(LFM_EINVAL, "bad argument");
+    Solver *s = h->impl.get();
+    for (int i = 0; i < nsteps; i++) s->step();
+    HIP_TRY(hipGetLastError());
+    return LFM_OK;
+}
+
+int lfm_synchronize(lfm_solver *h) {
+    if (!h) return fail(LFM_EINVAL, "null handle");
+    HIP_TRY(hipStreamSynchronize(h->impl->stream));
+    return LFM_OK;
+}
+
+int lfm_time_steps(lfm_solver *h, int nsteps, double *ms_out) {
+    if (!h || !ms_out || nsteps < 0) return fail(LFM_EINVAL, "bad argument");
+    Solver *s = h->impl.get();
+    HIP_TRY(hipEventRecord(s->ev0, s->stream));
+    for (int i = 0; i < nsteps; i++) s->step();
+    HIP_TRY(hipEventRecord(s->ev1, s->stream));
+    HIP_TRY(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    *ms_out = ms;
+    return LFM_OK;
+}
+
+int lfm_get_field(lfm_solver *h, int which, double *out) {
+    if (!h || !out) return fail(LFM_EINVAL, "null argument");
+    return h->impl->get_field(which, out);
+}
+
+int lfm_set_field(lfm_solver *h, int which, const double *in) {
+    if (!h || !in) return fail(LFM_EINVAL, "null argument");
+    return h->impl->set_field(which, in);
+}
+
+int lfm_get_residual(lfm_solver *h, double *res) {
+    if (!h || !res) return fail(LFM_EINVAL, "null argument");
+    Solver *s = h->impl.get();
+    HIP_TRY(hipMemcpyAsync(s->h_scalar + 16, s->d_res, (s->L.D + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < s->L.D + 2; k++) res[k] = s->h_scalar[16 + k];
+    return LFM_OK;
+}
+
+int lfm_check_finite(lfm_solver *h, int64_t *n_bad) {
+    if (!h || !n_bad) return fail(LFM_EINVAL, "null argument");
+    return h->impl->check_finite(n_bad);
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// solver_state.hpp -- the device-resident state behind one lfm_solver handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kernels_fp64.hpp"
+#include "lfm_hip.h"
+#include "mesh_builder.hpp"
+
+namespace lfm {
+
+// FP32-perturbation kernel arguments (defined in kernels_fp32p.hpp).
+struct KArgs32;
+
+int fail(int code, const std::string &msg);
+
+struct Solver {
+    lfm_params prm{};
+    SolverLayout L;
+    Gas gas{};
+    double A[5]{}, B[5]{};
+    int nstages = 0;
+    double dt = 0.0;
+    bool fp32 = false;
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *h_scalar = nullptr;   // pinned host scratch (64 doubles)
+    std::vector<void *> allocs;
+    int64_t device_bytes = 0;
+
+    // device scalars: [0] deltaT, [2..6] residual sums, [8] cfl, [9] dt min
+    double *d_scalar = nullptr;
+    double *d_res = nullptr, *d_cfl = nullptr, *d_dtmin = nullptr;
+    double *d_dt_ptr() { return d_scalar; }
+
+    KArgs k64{};
+    KArgs32 *k32 = nullptr;       // owned via allocs-free host block
+
+    double pending_A = 1.0;
+    int pending_rk = 0;
+    int64_t steps_done = 0;
+    bool res_valid = false;
+
+    ~Solver();
+
+    // stage pieces on the handle's stream
+    void bc();
+    void gradvis();
+    void flux();
+    void update(double A, double B, bool zero_dq, bool res, bool accumulate_res);
+    void cfl_dt(int mode, double param, double *out, double *dt_dst);
+    void step();
+    void zero_state_for_step();
+
+    int get_field(int which, double *out);
+    int set_field(int which, const double *in);
+    int check_finite(int64_t *n_bad);
+};
+
+int create_fp32p(Solver *s);
+
+}  // namespace lfm

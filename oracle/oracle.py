@@ -1,0 +1,181 @@
+"""ctypes binding of the CPU oracle (oracle/liblfm_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblfm_oracle.so")
+
+F_Q, F_DQ, F_DUDX, F_DTDX, F_TAUMC, F_SIGMAU, F_RHOGRAD, F_PGRAD, F_UGRAD, F_RES, F_VOLINV, F_SIGMA, \
+    F_CENTRE, F_VOLUME, F_SUBMESH = range(15)
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("n_points", C.c_int), ("points", C.c_void_p), ("n_faces", C.c_int),
+                ("face_off", C.c_void_p), ("face_pts", C.c_void_p), ("owner", C.c_void_p),
+                ("n_internal", C.c_int), ("neighbour", C.c_void_p), ("n_cells", C.c_int),
+                ("n_patches", C.c_int), ("patch_start", C.c_void_p), ("patch_size", C.c_void_p),
+                ("patch_role", C.c_void_p), ("patch_nbr", C.c_void_p), ("cell_submesh", C.c_void_p)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("dim", C.c_int), ("solver", C.c_int), ("rk_order", C.c_int), ("minmod", C.c_int),
+                ("adjust_dt", C.c_int), ("have_res", C.c_int), ("max_co", C.c_double), ("dt", C.c_double),
+                ("mol_weight", C.c_double), ("cp", C.c_double), ("mu", C.c_double), ("pr", C.c_double),
+                ("p_inf", C.c_double), ("T_inf", C.c_double), ("U_inf", C.c_double * 3),
+                ("Ls", C.c_double), ("k_sponge", C.c_double), ("mach", C.c_double)]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with gcc (ISO C99, no FMA contraction)."""
+    src = os.path.join(HERE, "lfm_oracle.c")
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", HERE])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.lfmo_create.restype = C.c_void_p
+        L.lfmo_create.argtypes = [C.POINTER(_Mesh), C.POINTER(_Params), C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_char_p, C.c_int]
+        L.lfmo_destroy.argtypes = [C.c_void_p]
+        L.lfmo_run.argtypes = [C.c_void_p, C.c_int]
+        L.lfmo_stage.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.lfmo_get_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.lfmo_ncomp.argtypes = [C.c_void_p, C.c_int]
+        L.lfmo_get_dt.restype = C.c_double
+        L.lfmo_get_dt.argtypes = [C.c_void_p]
+        L.lfmo_get_cfl.restype = C.c_double
+        L.lfmo_get_cfl.argtypes = [C.c_void_p]
+        L.lfmo_get_res.argtypes = [C.c_void_p, C.c_void_p]
+        L.lfmo_compute_dt.restype = C.c_double
+        L.lfmo_compute_dt.argtypes = [C.c_void_p, C.c_double]
+        L.lfmo_compute_cfl.restype = C.c_double
+        L.lfmo_compute_cfl.argtypes = [C.c_void_p, C.c_double]
+        L.lfmo_get_slots.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.lfmo_n_ghost.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+class Oracle:
+    """One oracle solve: mirrors Mesh<P,D>::initialize/initializeSolver + solve."""
+
+    def __init__(self, mesh, params: dict, p, T, U, alpha=None, cell_submesh=None):
+        L = lib()
+        self._keep = []
+
+        def arr(x, dt):
+            a = np.ascontiguousarray(x, dtype=dt)
+            self._keep.append(a)
+            return a
+
+        start, size, role, nbr = mesh.patch_arrays()
+        m = _Mesh()
+        m.n_points = mesh.points.shape[0]
+        m.points = _ptr(arr(mesh.points, np.float64))
+        m.n_faces = mesh.n_faces
+        m.face_off = _ptr(arr(mesh.face_off, np.int32))
+        m.face_pts = _ptr(arr(mesh.face_pts, np.int32))
+        m.owner = _ptr(arr(mesh.owner, np.int32))
+        m.n_internal = mesh.n_internal
+        m.neighbour = _ptr(arr(mesh.neighbour, np.int32))
+        m.n_cells = mesh.n_cells
+        m.n_patches = len(mesh.patches)
+        m.patch_start = _ptr(arr(start, np.int32))
+        m.patch_size = _ptr(arr(size, np.int32))
+        m.patch_role = _ptr(arr(role, np.int32))
+        m.patch_nbr = _ptr(arr(nbr, np.int32))
+        m.cell_submesh = _ptr(arr(cell_submesh, np.int32)) if cell_submesh is not None else None
+        pr = _Params()
+        for k, v in params.items():
+            if k == "U_inf":
+                for i in range(3):
+                    pr.U_inf[i] = float(v[i])
+            elif hasattr(pr, k):
+                setattr(pr, k, v)
+        self.dim = pr.dim
+        self.n_cells = mesh.n_cells
+        nc = mesh.n_cells
+        p = arr(p, np.float64)
+        T = arr(T, np.float64)
+        U = arr(np.asarray(U, np.float64).reshape(nc, 3), np.float64)
+        a = arr(alpha if alpha is not None else np.full(nc, 1e300), np.float64)
+        err = C.create_string_buffer(256)
+        self.h = L.lfmo_create(C.byref(m), C.byref(pr), _ptr(p), _ptr(T), _ptr(U), _ptr(a), err, 256)
+        if not self.h:
+            raise ValueError("oracle: " + err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lfmo_destroy(self.h)
+            self.h = None
+
+    def run(self, nsteps: int):
+        lib().lfmo_run(self.h, nsteps)
+
+    def stage(self, rk: int, begin_step: bool):
+        lib().lfmo_stage(self.h, rk, int(begin_step))
+
+    def field(self, which: int) -> np.ndarray:
+        nk = lib().lfmo_ncomp(self.h, which)
+        out = np.zeros((nk, self.n_cells))
+        lib().lfmo_get_field(self.h, which, _ptr(out))
+        return out
+
+    def q(self) -> np.ndarray:
+        return self.field(F_Q)
+
+    @property
+    def dt(self) -> float:
+        return lib().lfmo_get_dt(self.h)
+
+    @property
+    def cfl(self) -> float:
+        return lib().lfmo_get_cfl(self.h)
+
+    def res(self) -> np.ndarray:
+        out = np.zeros(5)
+        lib().lfmo_get_res(self.h, _ptr(out))
+        return out[: self.dim + 2]
+
+    def compute_dt(self, max_co: float) -> float:
+        return lib().lfmo_compute_dt(self.h, max_co)
+
+    def compute_cfl(self, dt: float) -> float:
+        return lib().lfmo_compute_cfl(self.h, dt)
+
+    def slots(self):
+        nc = self.n_cells
+        nface = np.zeros(nc, np.int32)
+        nvalid = np.zeros(nc, np.int32)
+        nbr = np.zeros(nc * 6, np.int32)
+        S = np.zeros(nc * 18)
+        d = np.zeros(nc * 18)
+        w = np.zeros(nc * 6)
+        lib().lfmo_get_slots(self.h, _ptr(nface), _ptr(nvalid), _ptr(nbr), _ptr(S), _ptr(d), _ptr(w))
+        return nface, nvalid, nbr.reshape(nc, 6), S.reshape(nc, 6, 3), d.reshape(nc, 6, 3), w.reshape(nc, 6)
+
+    @property
+    def n_ghost(self) -> int:
+        return lib().lfmo_n_ghost(self.h)
