@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X time step on the headline workload.
+
+Workload (default): 2D unstructured annulus around a unit cylinder,
+1,048,576 cells (256 rings of quads, 384 rings of triangles, 10 % seeded
+node jitter), uniform inflow M = 0.3 with 1e-4 pressure noise, M2 scheme
+with minmod AUSM pressure dissipation (solver 2), viscous terms, sponge of
+5 diameters from the outlet, RK5, constant dt.  Synthetic mesh and fields.
+
+A "step" is one full time step (5 RK stages) of every cell; the metric is
+cell-updates/s = cells x steps / s with the state resident in HBM.  The
+FP32-perturbation path is the headline value when it is built; the FP64
+path is timed beside it and its error is reported.
+
+Prints ONE JSON line (rank 0, one GPU).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(nc, nf, D, w_s, w_g):
+    """Algorithmic bytes of one RK stage: q and dq read + written, 1/V and
+    sigma per cell; S, d, w and the two cell indices per face."""
+    return nc * (4 * (D + 2) * w_s + 2 * w_g) + nf * ((2 * D + 1) * w_g + 8)
+
+
+def build_case(args):
+    from lfm_public_amd import cases
+    if args.workload == "cylinder2d":
+        return cases.cylinder_2d(n_theta=args.n_theta, n_quad=args.n_theta // 4, n_tri=3 * args.n_theta // 8)
+    if args.workload == "vortex2d":
+        return cases.isentropic_vortex(nx=args.n_theta // 4)
+    raise SystemExit(f"unknown workload {args.workload}")
+
+
+def time_path(case, precision, steps, warmup):
+    from lfm_public_amd import capi
+    g = capi.GpuSolver.from_case(case, precision=precision)
+    info = g.info()
+    g.run(warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    ms = g.time_steps(steps)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    prof, prof_total = g.profile_steps(max(1, min(steps, 10)))
+    return g, info, ms, wall, prof, prof_total
+
+
+def rel_errors(case, q, qref, D):
+    from lfm_public_amd import cases
+    gam = cases.gamma_of(case.params)
+    R = cases.gas_constant(case.params["mol_weight"])
+
+    def prim(qq):
+        rho = qq[0]
+        U = qq[1:D + 1] / rho
+        p = (qq[D + 1] - 0.5 * rho * (U * U).sum(0)) * (gam - 1)
+        return p, U, p / (rho * R)
+
+    p, U, T = prim(q)
+    pr, Ur, Tr = prim(qref)
+    inf = {"p": case.params["p_inf"], "T": case.params["T_inf"]}
+    out = {}
+    for name, a, b in (("p", p, pr), ("T", T, Tr)):
+        out[name] = {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                     "pert_l2": float(np.linalg.norm(a - b) / max(np.linalg.norm(b - inf[name]), 1e-300))}
+    for k in range(D):
+        uinf = case.params["U_inf"][k]
+        out[f"U{k}"] = {"rel_l2": float(np.linalg.norm(U[k] - Ur[k]) / max(np.linalg.norm(Ur[k]), 1e-300)),
+                        "pert_l2": float(np.linalg.norm(U[k] - Ur[k]) / max(np.linalg.norm(Ur[k] - uinf), 1e-300))}
+    return out
+
+
+def cpu_baseline(case, budget_s):
+    """The oracle (single-thread CPU restatement) on the same mesh, a bounded
+    number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    o = O.Oracle(case.mesh, case.params, case.p, case.T, case.U, case.alpha)
+    t0 = time.perf_counter()
+    o.run(1)
+    t1 = time.perf_counter() - t0
+    n = max(1, min(10, int(budget_s / max(t1, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    o.run(n)
+    dt = time.perf_counter() - t0
+    return {"value": case.n_cells * n / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (C restatement, 1 thread) on the full {case.n_cells}-cell mesh, {n} RK5 steps "
+                      f"after 1 warm-up step, FP64, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="cylinder2d")
+    ap.add_argument("--n-theta", type=int, default=1024)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by spatial "
+              f"decomposition but only the single-GPU partition is built and measured.", file=sys.stderr)
+        sys.exit(2)
+
+    from lfm_public_amd import capi
+    case = build_case(args)
+    D = case.params["dim"]
+    nc = case.n_cells
+    print(f"[bench] workload {case.name}: {nc} cells", file=sys.stderr, flush=True)
+
+    g64, info64, ms64, wall64, prof64, prof64_tot = time_path(case, capi.PREC_FP64, args.steps, args.warmup)
+    nf = info64.n_faces
+    rate64 = nc * args.steps / (ms64 * 1e-3)
+    print(f"[bench] FP64: {ms64 / args.steps:.3f} ms/step, {rate64:.3e} cell-updates/s", file=sys.stderr,
+          flush=True)
+    q64 = g64.q()
+    g64.close()
+
+    have32 = False
+    res32 = None
+    if not args.fp64_only:
+        try:
+            g32, info32, ms32, wall32, prof32, prof32_tot = time_path(case, capi.PREC_FP32P, args.steps,
+                                                                      args.warmup)
+            have32 = True
+        except capi.LfmError as e:
+            print(f"[bench] FP32-perturbation path unavailable: {e}", file=sys.stderr)
+    if have32:
+        rate32 = nc * args.steps / (ms32 * 1e-3)
+        # both paths advanced warmup + steps + profiled steps from the same state
+        q32 = g32.q()
+        err = rel_errors(case, q32, q64, D)
+        g32.close()
+        print(f"[bench] FP32P: {ms32 / args.steps:.3f} ms/step, {rate32:.3e} cell-updates/s", file=sys.stderr,
+              flush=True)
+
+    # roofline of the RK stage (the four stage kernels back to back)
+    def roof(prof, prof_tot, w, nsteps_prof):
+        stage_ms = sum(prof[k] for k in ("bc_fill", "grad_vis", "face_flux", "cell_update")) / (5 * nsteps_prof)
+        B = stage_bytes(nc, nf, D, w, w)
+        ach = B / (stage_ms * 1e-3) / 1e9
+        dom = max(("grad_vis", "face_flux", "cell_update"), key=lambda k: prof[k])
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "rk_stage (bc_fill + grad_vis + face_flux + cell_update)",
+                "stage_ms": round(stage_ms, 5), "algorithmic_bytes_per_stage": int(B),
+                "dominant_kernel": dom,
+                "kernel_ms_per_stage": {k: round(v / (5 * nsteps_prof), 5) for k, v in prof.items()}}
+
+    nprof = max(1, min(args.steps, 10))
+    if have32:
+        value, ms_step, dtype = rate32, ms32 / args.steps, "f32"
+        rl = roof(prof32, prof32_tot, 4, nprof)
+    else:
+        value, ms_step, dtype = rate64, ms64 / args.steps, "f64"
+        rl = roof(prof64, prof64_tot, 8, nprof)
+
+    cpu = None if args.no_cpu else cpu_baseline(case, args.cpu_budget)
+    out = {
+        "metric": "cell-updates/s FP64 vs FP32-perturbation at 1e-5 rel err; HBM GB/s; p99",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic",
+        "config": {"workload": f"{case.name}: 2D mixed quad/tri annulus, {nc} cells, {nf} faces, M=0.3, "
+                               f"solver 2 (M2 + minmod AUSM dissipation), viscous, sponge Ls=5D, RK5",
+                   "cells": nc, "faces": nf, "rk_stages": 5, "precision_paths": ["fp64"] + (["fp32p"] if have32 else [])},
+        "fp64": {"cell_updates_per_s": rate64, "ms_per_step": ms64 / args.steps,
+                 "roofline_stage": roof(prof64, prof64_tot, 8, nprof)},
+        "roofline": rl,
+        "cpu_baseline": cpu,
+    }
+    if have32:
+        out["fp32p"] = {"cell_updates_per_s": rate32, "ms_per_step": ms32 / args.steps,
+                        "speedup_vs_fp64": rate32 / rate64, "error_vs_fp64": err,
+                        "error_steps": args.warmup + args.steps + nprof}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

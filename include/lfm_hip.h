@@ -144,6 +144,11 @@ int lfm_synchronize(lfm_solver *s);
 /* Time n steps with HIP events on the handle's stream (ms). */
 int lfm_time_steps(lfm_solver *s, int nsteps, double *ms_out);
 
+/* Time n steps with HIP events around every kernel group on the handle's
+ * stream: ms_per_group[5] = bc fill, gradients+viscous, face fluxes,
+ * cell update (+ residual reduction), CFL/dt reduction; ms_total = whole. */
+int lfm_profile_steps(lfm_solver *s, int nsteps, double *ms_per_group, double *ms_total);
+
 /* Fields in original cell order, SoA [comp][cell], FP64 on the host. */
 int lfm_get_field(lfm_solver *s, int which, double *out);
 int lfm_set_field(lfm_solver *s, int which, const double *in);

@@ -27,7 +27,7 @@ EXPORTS = ("lfm_patch_role", "lfm_solver_create", "lfm_solver_destroy", "lfm_las
            "lfm_prepare_for_timestep", "lfm_prepare_for_rkstep", "lfm_set_boundary_conditions",
            "lfm_calc_gradients", "lfm_calc_vis", "lfm_one_rk_step", "lfm_compute_dt", "lfm_compute_cfl", "lfm_run",
            "lfm_synchronize", "lfm_time_steps", "lfm_get_field", "lfm_set_field", "lfm_get_residual",
-           "lfm_check_finite")
+           "lfm_check_finite", "lfm_profile_steps")
 
 
 class LfmError(RuntimeError):
@@ -86,6 +86,7 @@ def load() -> C.CDLL:
     L.lfm_compute_cfl.argtypes = [vp, dbl, C.POINTER(dbl)]
     L.lfm_run.argtypes = [vp, C.c_int]
     L.lfm_time_steps.argtypes = [vp, C.c_int, C.POINTER(dbl)]
+    L.lfm_profile_steps.argtypes = [vp, C.c_int, vp, C.POINTER(dbl)]
     L.lfm_get_field.argtypes = [vp, C.c_int, vp]
     L.lfm_set_field.argtypes = [vp, C.c_int, vp]
     L.lfm_get_residual.argtypes = [vp, vp]
@@ -218,6 +219,15 @@ class GpuSolver:
         ms = C.c_double()
         _check(load().lfm_time_steps(self.h, nsteps, C.byref(ms)))
         return ms.value
+
+    PROFILE_GROUPS = ("bc_fill", "grad_vis", "face_flux", "cell_update", "cfl_dt")
+
+    def profile_steps(self, nsteps: int):
+        """(ms per kernel group summed over nsteps, total ms) from HIP events."""
+        per = np.zeros(8)
+        tot = C.c_double()
+        _check(load().lfm_profile_steps(self.h, nsteps, _ptr(per), C.byref(tot)))
+        return dict(zip(self.PROFILE_GROUPS, per[:5].tolist())), tot.value
 
     def info(self) -> Info:
         i = Info()

@@ -252,19 +252,37 @@ void Solver::cfl_dt(int mode, double param, double *out, double *dt_dst) {
     }
 }
 
+void Solver::prof_mark(int grp, bool start) {
+    if (!prof) return;
+    if (prof_n >= prof_ev.size()) return;
+    (void)hipEventRecord(prof_ev[prof_n], stream);
+    prof_grp[prof_n] = start ? grp : -1 - grp;
+    prof_n++;
+}
+
 // one full time step (Mesh::solve loop body), no host synchronisation
 void Solver::step() {
     const bool res = prm.have_res != 0;
     for (int rk = 0; rk < nstages; rk++) {
+        prof_mark(0, true);
         bc();
+        prof_mark(0, false);
+        prof_mark(1, true);
         gradvis();
+        prof_mark(1, false);
+        prof_mark(2, true);
         flux();
+        prof_mark(2, false);
+        prof_mark(3, true);
         update(A[rk], B[rk], rk == 0, res && rk == 0, false);
+        prof_mark(3, false);
     }
+    prof_mark(4, true);
     if (prm.adjust_dt)
         cfl_dt(0, prm.max_co, d_dtmin, d_dt_ptr());
     else
         cfl_dt(1, dt, d_cfl, nullptr);
+    prof_mark(4, false);
     steps_done++;
     if (res) res_valid = true;
 }
@@ -460,6 +478,37 @@ int lfm_time_steps(lfm_solver *h, int nsteps, double *ms_out) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     *ms_out = ms;
+    return LFM_OK;
+}
+
+int lfm_profile_steps(lfm_solver *h, int nsteps, double *ms_per_group, double *ms_total) {
+    if (!h || !ms_per_group || !ms_total || nsteps < 1) return fail(LFM_EINVAL, "bad argument");
+    Solver *s = h->impl.get();
+    const size_t need = (size_t)nsteps * (s->nstages * 8 + 2);
+    for (hipEvent_t e : s->prof_ev) (void)hipEventDestroy(e);
+    s->prof_ev.assign(need, nullptr);
+    s->prof_grp.assign(need, 0);
+    for (size_t i = 0; i < need; i++) HIP_TRY(hipEventCreate(&s->prof_ev[i]));
+    s->prof_n = 0;
+    s->prof = true;
+    HIP_TRY(hipEventRecord(s->ev0, s->stream));
+    for (int i = 0; i < nsteps; i++) s->step();
+    HIP_TRY(hipEventRecord(s->ev1, s->stream));
+    s->prof = false;
+    HIP_TRY(hipEventSynchronize(s->ev1));
+    for (int g = 0; g < Solver::kProfGroups; g++) ms_per_group[g] = 0.0;
+    for (size_t i = 0; i + 1 < s->prof_n; i += 2) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->prof_ev[i], s->prof_ev[i + 1]));
+        const int g = s->prof_grp[i];
+        if (g >= 0 && g < Solver::kProfGroups) ms_per_group[g] += ms;
+    }
+    float tot = 0.f;
+    HIP_TRY(hipEventElapsedTime(&tot, s->ev0, s->ev1));
+    *ms_total = tot;
+    for (hipEvent_t e : s->prof_ev) (void)hipEventDestroy(e);
+    s->prof_ev.clear();
+    s->prof_grp.clear();
     return LFM_OK;
 }
 

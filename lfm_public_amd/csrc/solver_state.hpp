@@ -41,6 +41,14 @@ struct Solver {
     KArgs k64{};
     KArgs32 *k32 = nullptr;       // owned via allocs-free host block
 
+    // per-kernel-group timing with HIP events (lfm_profile_steps)
+    static constexpr int kProfGroups = 5;  // bc, grad_vis, face_flux, cell_update, cfl_dt
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;       // pairs (start, stop)
+    std::vector<int> prof_grp;
+    size_t prof_n = 0;
+    void prof_mark(int grp, bool start);
+
     double pending_A = 1.0;
     int pending_rk = 0;
     int64_t steps_done = 0;
