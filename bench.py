@@ -129,7 +129,10 @@ def main():
     print(f"[bench] FP64: {ms64 / args.steps:.3f} ms/step, {rate64:.3e} cell-updates/s", file=sys.stderr,
           flush=True)
     q64 = g64.q()
+    bad64 = g64.n_nonfinite()
     g64.close()
+    if bad64:
+        raise SystemExit(f"FP64 state went non-finite ({bad64} values): the workload's dt is unstable")
 
     have32 = False
     res32 = None

@@ -155,6 +155,9 @@ int lfm_set_field(lfm_solver *s, int which, const double *in);
 /* Sum of squared residuals of the last step's stage 0 (D+2), as printed by
  * Mesh::solve (mesh_solver.cpp:763-793, before the square root). */
 int lfm_get_residual(lfm_solver *s, double *res);
+/* FP64 conservation integrals sum_c V_c q_c (D+2 values: mass, momentum,
+ * energy), for either precision path. */
+int lfm_conservation(lfm_solver *s, double *out);
 /* Number of non-finite conservatives in the state (0 = healthy). */
 int lfm_check_finite(lfm_solver *s, int64_t *n_bad);
 

@@ -27,7 +27,7 @@ EXPORTS = ("lfm_patch_role", "lfm_solver_create", "lfm_solver_destroy", "lfm_las
            "lfm_prepare_for_timestep", "lfm_prepare_for_rkstep", "lfm_set_boundary_conditions",
            "lfm_calc_gradients", "lfm_calc_vis", "lfm_one_rk_step", "lfm_compute_dt", "lfm_compute_cfl", "lfm_run",
            "lfm_synchronize", "lfm_time_steps", "lfm_get_field", "lfm_set_field", "lfm_get_residual",
-           "lfm_check_finite", "lfm_profile_steps")
+           "lfm_check_finite", "lfm_profile_steps", "lfm_conservation")
 
 
 class LfmError(RuntimeError):
@@ -91,6 +91,7 @@ def load() -> C.CDLL:
     L.lfm_set_field.argtypes = [vp, C.c_int, vp]
     L.lfm_get_residual.argtypes = [vp, vp]
     L.lfm_check_finite.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.lfm_conservation.argtypes = [vp, vp]
     _lib = L
     return L
 
@@ -254,6 +255,12 @@ class GpuSolver:
     def residual(self) -> np.ndarray:
         out = np.zeros(5)
         _check(load().lfm_get_residual(self.h, _ptr(out)))
+        return out[: self.dim + 2]
+
+    def conservation(self) -> np.ndarray:
+        """FP64 integrals sum_c V_c q_c (mass, momentum, energy)."""
+        out = np.zeros(5)
+        _check(load().lfm_conservation(self.h, _ptr(out)))
         return out[: self.dim + 2]
 
     def n_nonfinite(self) -> int:

@@ -132,9 +132,11 @@ def cylinder_2d(n_theta=512, n_quad=None, n_tri=None, mach=0.3, noise=1e-4, seed
     alpha = np.hypot(cc[:, 0] - nx_, cc[:, 1] - ny_)
     Ls = Ls_diam * 1.0
     prm["Ls"] = Ls
-    # constant dt from the smallest cell
-    h_min = 0.5 * 2 * math.pi / n_theta
-    prm["dt"] = cfl * h_min / (U_inf + c_inf)
+    # constant dt from the smallest (jittered) wall cell: convective CFL and
+    # the explicit viscous limit nu dt / h^2 <= 1/4
+    h_min = (1.0 - jitter) * 0.5 * 2 * math.pi / n_theta
+    nu = prm["mu"] / rho_inf
+    prm["dt"] = cfl * min(h_min / (U_inf + c_inf), 0.25 * h_min * h_min / nu)
     del r
     return Case(f"cylinder2d_{nc}", mesh, prm, p, T, U, alpha)
 

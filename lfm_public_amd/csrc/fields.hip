@@ -16,14 +16,22 @@ namespace lfm {
             return fail(LFM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
     } while (0)
 
-int create_fp32p(Solver *) { return fail(LFM_EINVAL, "FP32-perturbation path not available in this build"); }
-
 // ISolver::prepare_for_timestep: dq = 0, RES = 0
+__global__ void __launch_bounds__(256) fill_zero32(float *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
 void Solver::zero_state_for_step() {
     const size_t n = (size_t)(L.D + 2) * L.nc;
     const int nb = (int)((n + 255) / 256);
-    hipLaunchKernelGGL(fill_zero, dim3(nb), dim3(256), 0, stream, k64.dq, n);
-    hipLaunchKernelGGL(fill_zero, dim3(nb), dim3(256), 0, stream, k64.res_cell, n);
+    if (fp32) {
+        hipLaunchKernelGGL(fill_zero32, dim3(nb), dim3(256), 0, stream, k32->dq, n);
+        hipLaunchKernelGGL(fill_zero32, dim3(nb), dim3(256), 0, stream, k32->res_cell, n);
+    } else {
+        hipLaunchKernelGGL(fill_zero, dim3(nb), dim3(256), 0, stream, k64.dq, n);
+        hipLaunchKernelGGL(fill_zero, dim3(nb), dim3(256), 0, stream, k64.res_cell, n);
+    }
     pending_A = 1.0;
 }
 
@@ -79,7 +87,7 @@ int Solver::get_field(int which, double *out) {
         }
         return LFM_OK;
     }
-    if (fp32) return fail(LFM_EINVAL, "field not available for the FP32 path");
+    if (fp32) return get_field32(this, which, out);
     FieldRef r = field_ref64(this, which);
     if (!r.ptr) return fail(LFM_EINVAL, "unknown or unallocated field");
     std::vector<double> tmp(nc);
@@ -94,7 +102,20 @@ int Solver::get_field(int which, double *out) {
 int Solver::set_field(int which, const double *in) {
     const size_t nc = L.nc;
     HIP_TRY(hipStreamSynchronize(stream));
-    if (fp32) return fail(LFM_EINVAL, "field not settable for the FP32 path");
+    if (fp32) {
+        if (which != LFM_FIELD_Q) return fail(LFM_EINVAL, "only q is settable on the FP32 path");
+        KArgs32 &a = *k32;
+        double qinf[5] = {a.rho_inf_d, a.m_inf_d[0], a.m_inf_d[1], a.m_inf_d[2], 0.0};
+        qinf[L.D + 1] = a.rhoE_inf_d;
+        std::vector<float> tmp(nc);
+        for (int k = 0; k < L.D + 2; k++) {
+            const double *src = in + k * nc;
+            for (size_t ci = 0; ci < nc; ci++) tmp[ci] = (float)(src[L.orig[ci]] - qinf[k]);
+            HIP_TRY(hipMemcpy(a.q + k * ((size_t)L.nc + L.ng), tmp.data(), nc * sizeof(float),
+                              hipMemcpyHostToDevice));
+        }
+        return LFM_OK;
+    }
     FieldRef r = field_ref64(this, which);
     if (!r.ptr || (which != LFM_FIELD_Q && which != LFM_FIELD_DQ)) return fail(LFM_EINVAL, "field not settable");
     std::vector<double> tmp(nc);
@@ -111,7 +132,9 @@ int Solver::check_finite(int64_t *n_bad) {
     HIP_TRY(hipMalloc((void **)&d, sizeof(unsigned long long)));
     HIP_TRY(hipMemsetAsync(d, 0, sizeof(unsigned long long), stream));
     const int nb = (L.nc + 255) / 256;
-    if (!fp32) {
+    if (fp32) {
+        nonfinite32(this, d);
+    } else {
         if (L.D == 2)
             hipLaunchKernelGGL(count_nonfinite<2>, dim3(nb), dim3(256), 0, stream, k64.q, L.nc, L.nc + L.ng, d);
         else

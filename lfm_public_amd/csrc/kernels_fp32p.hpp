@@ -1,4 +1,23 @@
-// kernels_fp32p.hpp -- FP32 deviation-from-freestream time step (declarations).
+// kernels_fp32p.hpp -- FP32 time step on deviations from the freestream.
+//
+// State: q' = q - q_inf in FP32 (q_inf = rho_inf, rho_inf U_inf, rho_inf E_inf
+// held as constants).  Every face flux is evaluated as F(q_f).S - F(q_inf).S
+// with the deviation formed algebraically, never by subtracting two
+// freestream-sized numbers:
+//   u' = (m' - u_inf rho') / rho
+//   p' = (g-1) [ (rhoE)' - 1/2 (rho' |u|^2 + rho_inf u'.(2 u_inf + u')) ]
+//   Rpsi' = (p' - Rpsi_inf rho') / rho
+// Subtracting F(q_inf).S face by face leaves every closed cell's sum unchanged
+// in exact arithmetic (sum of S over a closed cell is zero) and removes the
+// FP32 closure error of the S vectors times the freestream flux.  Green-Gauss
+// gradients are taken of the deviations for the same reason.  dt, its
+// min-reduction, the CFL and the conservation integrals stay FP64.
+//
+// M1 (cfd_v0.cpp:2589-2675): both reconstructed states are the same linear
+// interpolation (the reference forms rho- with weight 1-w on the swapped pair),
+// so the central-upwind combination [psi+ F+ - psi- F- + psi+ psi- (q- - q+)]
+// / (psi+ - psi-) equals F+ exactly; this path evaluates F+ directly and so
+// carries none of the rounding-level dissipation the FP64 form contains.
 #pragma once
 
 #include "solver_state.hpp"
@@ -6,18 +25,684 @@
 namespace lfm {
 
 struct KArgs32 {
-    int dummy;
+    int nc, nsm0, ng, nf, ntot;
+    const int32_t *fP, *fN;
+    const float *S, *d, *w;
+    const int32_t *g_off, *g_ent, *s_off, *s_ent;
+    const int32_t *ghost_cell, *ghost_role;
+    const float *vinv, *sigma;
+    float *q, *dq, *res_cell;
+    float *dudx, *dTdx, *tauMC, *sigmaU;
+    float *rho_grad, *p_grad, *U_grad;
+    float *frhs;
+    const double *dt;
+    double *partial;
+    const double *vol_d;  // FP64 cell volumes (conservation integrals)
+    float gamma, gm1, gog, R_inv, mu, cp, pr_inv;
+    float rho_inf, p_inf, T_inf, Rpsi_inf, E_inf, H_inf, G_inf, c_inf;
+    float uinf[3], minf[3];
+    double rho_inf_d, m_inf_d[3], rhoE_inf_d;
+};
+
+#define LERP32(w, a, b) ((w) * (a) + (1.0f - (w)) * (b))
+
+template <int D>
+struct Prim32 {
+    float rp, rho, ri, up[D], u[D], pp, Rp, R;
 };
 
 template <int D>
-void launch_bc32(Solver *) {}
+__device__ __forceinline__ Prim32<D> prim32(const KArgs32 &a, const float *q) {
+    Prim32<D> s;
+    s.rp = q[0];
+    s.rho = a.rho_inf + s.rp;
+    s.ri = 1.0f / s.rho;
+    float u2 = 0.f, du2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        s.up[k] = (q[k + 1] - a.uinf[k] * s.rp) * s.ri;
+        s.u[k] = a.uinf[k] + s.up[k];
+        u2 += s.u[k] * s.u[k];
+        du2 += s.up[k] * (2.f * a.uinf[k] + s.up[k]);
+    }
+    s.pp = a.gm1 * (q[D + 1] - 0.5f * (s.rp * u2 + a.rho_inf * du2));
+    s.Rp = (s.pp - a.Rpsi_inf * s.rp) * s.ri;
+    s.R = a.Rpsi_inf + s.Rp;
+    return s;
+}
+
 template <int D>
-void launch_gradvis32(Solver *) {}
+__device__ __forceinline__ void load_q32(const KArgs32 &a, int i, float *q) {
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) q[k] = a.q[(size_t)k * a.ntot + i];
+}
+
 template <int D>
-void launch_flux32(Solver *) {}
+__device__ __forceinline__ float dot32(const float *x, const float *y) {
+    float s = x[0] * y[0];
+#pragma unroll
+    for (int k = 1; k < D; k++) s += x[k] * y[k];
+    return s;
+}
+
+__device__ __forceinline__ float sgn32(float x) { return x < 0.0f ? -1.0f : 1.0f; }
+
+// interp_minmod (api/cfdv0_solver.h:311-370) on deviations: phi' differences
+// equal phi differences, and the weights sum to one, so phi_inf is added back
+// by the caller.
 template <int D>
-void launch_update32(Solver *, double, double, bool, bool, bool) {}
+__device__ __forceinline__ float minmod32(float cphi, float aphi, const float *grad, const float *d, float wl,
+                                          float flux) {
+    const float gradf = aphi - cphi + 1.0e-30f;
+    float gradcf = 0.0f;
+#pragma unroll
+    for (int k = 0; k < D; k++) gradcf += d[k] * grad[k];
+    float r;
+    if (fabsf(gradcf) >= 1000.0f * fabsf(gradf))
+        r = 2.0f * 1000.0f * sgn32(gradcf) * sgn32(gradf) - 1.0f;
+    else
+        r = 2.0f * (gradcf / gradf) - 1.0f;
+    const float m1 = (1.0f < r) ? 1.0f : r;
+    const float lim = (m1 < 0.0f) ? 0.0f : m1;
+    const float wt = lim * wl + (1.0f - lim) * (1.0f + flux) * 0.5f;
+    return wt * cphi + (1.0f - wt) * aphi;
+}
+
+__device__ __forceinline__ float p5pos32(float M, float alpha) {
+    const float M2Pos = 0.25f * (M + 1.0f) * (M + 1.0f);
+    const float M2Neg = -0.25f * (M - 1.0f) * (M - 1.0f);
+    if (fabsf(M) < 1.f) return M2Pos * ((2.0f - M) - 16.0f * alpha * M * M2Neg);
+    return 0.5f * (M + fabsf(M)) / M;
+}
+
+__device__ __forceinline__ float p5neg32(float M, float alpha) {
+    const float M2Pos = 0.25f * (M + 1.0f) * (M + 1.0f);
+    const float M2Neg = -0.25f * (M - 1.0f) * (M - 1.0f);
+    if (fabsf(M) < 1.f) return M2Neg * ((-2.0f - M) + 16.0f * alpha * M * M2Pos);
+    return 0.5f * (M - fabsf(M)) / M;
+}
+
+// ---------------------------------------------------------------------------
+// ghost states (cfd_v0.cpp:1010-1133) in deviation form
+// ---------------------------------------------------------------------------
 template <int D>
-void launch_cfl_dt32(Solver *, int, double, double *, double *) {}
+__global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.ng) return;
+    const int c = a.ghost_cell[g];
+    const int role = a.ghost_role[g];
+    float qi[D + 2], qg[D + 2];
+    load_q32<D>(a, c, qi);
+    if (role == 1) {  // wall: rho, -rhoU, rhoE
+        qg[0] = qi[0];
+#pragma unroll
+        for (int k = 0; k < D; k++) qg[k + 1] = -qi[k + 1] - 2.f * a.minf[k];
+        qg[D + 1] = qi[D + 1];
+    } else if (role == 2 || role == 3) {
+        const Prim32<D> s = prim32<D>(a, qi);
+        if (role == 2) {  // inlet: T = T_inf, U = U_inf, p = p_int
+            const float rpg = s.rp + s.rho * (s.Rp * a.R_inv) / a.T_inf;
+            qg[0] = rpg;
+            float u2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                qg[k + 1] = rpg * a.uinf[k];
+                u2 += a.uinf[k] * a.uinf[k];
+            }
+            qg[D + 1] = s.pp / a.gm1 + 0.5f * rpg * u2;
+        } else {  // outlet: p = p_inf, U = U_int, rho = rho_int p_inf / p_int
+            const float pint = a.p_inf + s.pp;
+            const float rpg = (s.rp * a.p_inf - a.rho_inf * s.pp) / pint;
+            const float rhog = a.rho_inf + rpg;
+            float u2 = 0.f, du2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                qg[k + 1] = rpg * a.uinf[k] + rhog * s.up[k];
+                u2 += s.u[k] * s.u[k];
+                du2 += s.up[k] * (2.f * a.uinf[k] + s.up[k]);
+            }
+            qg[0] = rpg;
+            qg[D + 1] = 0.5f * (rpg * u2 + a.rho_inf * du2);
+        }
+    } else {
+        return;  // unset patch: ghost q = 0, i.e. q' = -q_inf (set at creation)
+    }
+    const int gi = a.nc + g;
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + gi] = qg[k];
+}
+
+// ---------------------------------------------------------------------------
+// Green-Gauss of deviations + viscous tensors, per cell
+// ---------------------------------------------------------------------------
+template <int D, int GRAD>
+__global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nc) return;
+    const float vinv = a.vinv[c];
+    float dudx[D][D], dTdx[D], rg[D], pg[D], ug[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        dTdx[i] = rg[i] = pg[i] = 0.f;
+#pragma unroll
+        for (int j = 0; j < D; j++) dudx[i][j] = ug[i][j] = 0.f;
+    }
+    const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
+    for (int e = e0; e < e1; e++) {
+        const int ent = a.g_ent[e];
+        const bool isP = ent >= 0;
+        const int f = isP ? ent : ~ent;
+        const int P = a.fP[f], N = a.fN[f];
+        const float w = a.w[f];
+        float S[D], qP[D + 2], qN[D + 2];
+#pragma unroll
+        for (int k = 0; k < D; k++) S[k] = a.S[(size_t)k * a.nf + f];
+        load_q32<D>(a, P, qP);
+        load_q32<D>(a, N, qN);
+        const Prim32<D> sp = prim32<D>(a, qP), sn = prim32<D>(a, qN);
+        float sv[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) sv[k] = isP ? S[k] * vinv : -S[k] * vinv;
+        float fU[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) fU[k] = LERP32(w, sp.up[k], sn.up[k]);
+        const float fT = LERP32(w, sp.Rp, sn.Rp) * a.R_inv;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) dudx[i][j] += fU[i] * sv[j];
+            dTdx[i] += fT * sv[i];
+        }
+        if (GRAD == 2) {
+            const float rf = LERP32(w, sp.rp, sn.rp);
+            const float pf = LERP32(w, sp.pp, sn.pp);
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+                pg[i] += pf * sv[i];
+                rg[i] += rf * sv[i];
+#pragma unroll
+                for (int j = 0; j < D; j++) ug[j][i] += fU[j] * sv[i];
+            }
+        }
+    }
+    float qc[D + 2];
+    load_q32<D>(a, c, qc);
+    const Prim32<D> sc = prim32<D>(a, qc);
+    const float mu = a.mu;
+    float tau[D][D];
+#pragma unroll
+    for (int p = 0; p < D; p++) {
+        tau[p][p] = 2.0f * dudx[p][p];
+#pragma unroll
+        for (int b1 = p + 1; b1 < D + p; b1++) {
+            const int b = b1 % D;
+            tau[p][b] = mu * (dudx[p][b] + dudx[b][p]);
+            tau[p][p] -= dudx[b][b];
+        }
+        tau[p][p] *= 2.0f / 3.0f * mu;
+    }
+    float diag = 0.f;
+#pragma unroll
+    for (int p = 0; p < D; p++) diag -= dudx[p][p];
+    diag *= mu * 2.0f / 3.0f;
+    const size_t nt = a.ntot;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        a.sigmaU[i * nt + c] = dot32<D>(sc.u, tau[i]);
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            float t = mu * dudx[j][i];
+            if (i == j) t += diag;
+            a.tauMC[(i * D + j) * nt + c] = t;
+            a.dudx[(i * D + j) * nt + c] = dudx[i][j];
+        }
+        a.dTdx[i * nt + c] = dTdx[i];
+    }
+    if (GRAD == 2) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            a.rho_grad[i * nt + c] = rg[i];
+            a.p_grad[i * nt + c] = pg[i];
+#pragma unroll
+            for (int j = 0; j < D; j++) a.U_grad[(i * D + j) * nt + c] = ug[i][j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// face fluxes minus the freestream flux, one thread per face
+// ---------------------------------------------------------------------------
+template <int D, int SOLVER>
+__global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    const int P = a.fP[f], N = a.fN[f];
+    const bool ghost = N >= a.nc;
+    const float w = a.w[f];
+    const size_t nt = a.ntot;
+    float S[D], d[D], qP[D + 2], qN[D + 2];
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        S[k] = a.S[(size_t)k * a.nf + f];
+        d[k] = a.d[(size_t)k * a.nf + f];
+    }
+    load_q32<D>(a, P, qP);
+    load_q32<D>(a, N, qN);
+    const Prim32<D> sp = prim32<D>(a, qP), sn = prim32<D>(a, qN);
+    float Sm2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; k++) Sm2 += S[k] * S[k];
+    const float Sm = sqrtf(Sm2);
+    float phi_inf = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; k++) phi_inf += a.uinf[k] * S[k];
+    float rhs[D + 2];
+
+    if (SOLVER == 0) {
+        // reconstructed state (cfd_v0.cpp:2590-2637): conservatives, Rpsi
+        // (hence p and e = Rpsi/(g-1)) interpolated linearly
+        const float rp = LERP32(w, qP[0], qN[0]);
+        const float rho = a.rho_inf + rp;
+        const float ri = 1.f / rho;
+        float mp[D], up[D];
+        float phip = 0.f, du2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            mp[k] = LERP32(w, qP[k + 1], qN[k + 1]);
+            up[k] = (mp[k] - a.uinf[k] * rp) * ri;
+            phip += up[k] * S[k];
+            du2 += up[k] * (2.f * a.uinf[k] + up[k]);
+        }
+        const float phi = phi_inf + phip;
+        const float Rp = LERP32(w, sp.Rp, sn.Rp);
+        const float pp = rp * a.Rpsi_inf + rho * Rp;
+        // G = rhoE + p = rho (g/(g-1) Rpsi + |u|^2/2), e = Rpsi/(g-1) per cell
+        float uinf2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; k++) uinf2 += a.uinf[k] * a.uinf[k];
+        const float Gdev = rp * (a.gog * a.Rpsi_inf + 0.5f * uinf2) + rho * (a.gog * Rp + 0.5f * du2);
+        float mS = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; k++) mS += mp[k] * S[k];
+        rhs[0] = -mS;
+#pragma unroll
+        for (int k = 0; k < D; k++) rhs[k + 1] = -(phip * a.minf[k] + phi * mp[k] + pp * S[k]);
+        rhs[D + 1] = -(phip * a.G_inf + phi * Gdev);
+    } else {
+        // midpoint averages (cfd_v0.cpp:1968-2023)
+        const float rp = 0.5f * (qP[0] + qN[0]);
+        const float rho = a.rho_inf + rp;
+        const float ri = 1.f / rho;
+        float mp[D], up[D];
+        float phip = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            mp[k] = 0.5f * (qP[k + 1] + qN[k + 1]);
+            up[k] = (mp[k] - a.uinf[k] * rp) * ri;
+            phip += up[k] * S[k];
+        }
+        const float phi = phi_inf + phip;
+        const float Rp = 0.5f * (sp.Rp + sn.Rp);
+        float pp = rp * a.Rpsi_inf + rho * Rp;
+        // H = rhoE/rho + Rpsi = g/(g-1) Rpsi + |u|^2/2 per cell, averaged
+        float duP = 0.f, duN = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            duP += sp.up[k] * (2.f * a.uinf[k] + sp.up[k]);
+            duN += sn.up[k] * (2.f * a.uinf[k] + sn.up[k]);
+        }
+        const float Hp = 0.5f * ((a.gog * sp.Rp + 0.5f * duP) + (a.gog * sn.Rp + 0.5f * duN));
+        const float Gdev = rp * a.H_inf + rho * Hp;
+        if (SOLVER == 2) {
+            const float cP = sqrtf(a.gamma * sp.R), cN = sqrtf(a.gamma * sn.R);
+            const float cavg = 0.5f * (cP + cN);
+            float cdiv = 0.f, ndiv = 0.f, ccurl = 0.f, ncurl = 0.f, unP = 0.f, unN = 0.f;
+            float cdu[D][D], ndu[D][D];
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) {
+                    cdu[i][j] = a.dudx[(i * D + j) * nt + P];
+                    ndu[i][j] = a.dudx[(i * D + j) * nt + N];
+                }
+#pragma unroll
+            for (int p = 0; p < D; p++) {
+                cdiv += cdu[p][p];
+                ndiv += ndu[p][p];
+#pragma unroll
+                for (int b = p + 1; b < D; b++) {
+                    ccurl += (cdu[p][b] - cdu[b][p]) * (cdu[p][b] - cdu[b][p]);
+                    ncurl += (ndu[p][b] - ndu[b][p]) * (ndu[p][b] - ndu[b][p]);
+                }
+                float gc[D], gn[D];
+#pragma unroll
+                for (int k = 0; k < D; k++) {
+                    gc[k] = a.U_grad[(p * D + k) * nt + P];
+                    gn[k] = a.U_grad[(p * D + k) * nt + N];
+                }
+                unP += minmod32<D>(sp.up[p], sn.up[p], gc, d, w, 1.0f) * S[p];
+                unN += minmod32<D>(sp.up[p], sn.up[p], gn, d, w, -1.0f) * S[p];
+            }
+            const float cth0 = -(cdiv / sqrtf(cdiv * cdiv + ccurl + 4e-2f));
+            const float nth0 = -(ndiv / sqrtf(ndiv * ndiv + ncurl + 4e-2f));
+            const float tha = 0.5f * (((cth0 < 0.f) ? 0.f : cth0) + ((nth0 < 0.f) ? 0.f : nth0));
+            float gc[D], gn[D], hc[D], hn[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                gc[k] = a.rho_grad[k * nt + P];
+                gn[k] = a.rho_grad[k * nt + N];
+                hc[k] = a.p_grad[k * nt + P];
+                hn[k] = a.p_grad[k * nt + N];
+            }
+            const float rhoPos = a.rho_inf + minmod32<D>(sp.rp, sn.rp, gc, d, w, 1.0f);
+            const float rhoNeg = a.rho_inf + minmod32<D>(sp.rp, sn.rp, gn, d, w, -1.0f);
+            const float pPd = minmod32<D>(sp.pp, sn.pp, hc, d, w, 1.0f);
+            const float pNd = minmod32<D>(sp.pp, sn.pp, hn, d, w, -1.0f);
+            const float unPf = phi_inf + unP, unNf = phi_inf + unN;
+            const float Msq = (unPf * unPf + unNf * unNf) / (2.f * cavg * cavg * Sm2);
+            const float MP = unPf / (Sm * cavg);
+            const float MN = unNf / (Sm * cavg);
+            const float Minf = 0.2f;
+            const float mx = (Msq < Minf * Minf) ? Minf * Minf : Msq;
+            const float M0 = sqrtf((mx < 1.0f) ? mx : 1.0f);
+            const float fa = M0 * (2.0f - M0);
+            const float alpha = 3.0f * (5.0f * fa * fa - 4.0f) / 16.0f;
+            const float dN = p5pos32(MN, alpha) - p5neg32(MN, alpha);
+            const float dP = p5pos32(MP, alpha) - p5neg32(MP, alpha);
+            // p_half = pN dN - pP dP = p_inf (dN - dP) + pN' dN - pP' dP.  For
+            // |M| < 1, P5+(M) - P5-(M) = M (3 - M^2)/2 + 2 alpha M (1 - M^2)^2, so
+            // dN - dP is formed as (MN - MP) times its divided difference, with
+            // MN - MP from the normal-velocity deviations (no cancellation).
+            float dNP;
+            if (fabsf(MN) < 1.f && fabsf(MP) < 1.f) {
+                const float s2 = MN * MN + MN * MP + MP * MP;
+                const float s4 = MN * MN * MN * MN + MN * MN * MN * MP + MN * MN * MP * MP + MN * MP * MP * MP +
+                                 MP * MP * MP * MP;
+                const float dd = 0.5f * (3.f - s2) + 2.f * alpha * (1.f - 2.f * s2 + s4);
+                dNP = ((unN - unP) / (Sm * cavg)) * dd;
+            } else {
+                dNP = dN - dP;
+            }
+            const float phalf = a.p_inf * dNP + pNd * dN - pPd * dP;
+            const float pu = -0.75f * p5pos32(MP, alpha) * p5neg32(MN, alpha) * (rhoPos + rhoNeg) * cavg * fa *
+                             (unN - unP);
+            pp += tha * (pu - 0.5f * phalf);
+        }
+        float mS = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; k++) mS += mp[k] * S[k];
+        rhs[0] = -mS;
+#pragma unroll
+        for (int k = 0; k < D; k++) rhs[k + 1] = -(phip * a.minf[k] + phi * mp[k] + pp * S[k]);
+        rhs[D + 1] = -(phip * a.G_inf + phi * Gdev);
+    }
+
+    // viscous terms (cfd_v0.cpp:2682-2790) on velocity / temperature deviations
+    const float mu = a.mu;
+    float dmag2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < D; k++) dmag2 += d[k] * d[k];
+    const float dmi = 1.f / sqrtf(dmag2);
+    float dn[D], du[D][D], dT[D], tau[D][D], tmc[D][D];
+#pragma unroll
+    for (int k = 0; k < D; k++) dn[k] = d[k] * dmi;
+    const float cT = sp.Rp * a.R_inv, aT = sn.Rp * a.R_inv;
+    if (ghost) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) du[i][j] = (sn.up[i] - sp.up[i]) * dn[j] * dmi;
+            dT[i] = (aT - cT) * dn[i] * dmi;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++)
+                du[i][j] = LERP32(w, a.dudx[(i * D + j) * nt + P], a.dudx[(i * D + j) * nt + N]);
+            dT[i] = LERP32(w, a.dTdx[i * nt + P], a.dTdx[i * nt + N]);
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < D; p++) {
+        tau[p][p] = 2.0f * du[p][p];
+#pragma unroll
+        for (int b1 = p + 1; b1 < D + p; b1++) {
+            const int b = b1 % D;
+            tau[p][b] = mu * (du[p][b] + du[b][p]);
+            tau[p][p] -= du[b][b];
+        }
+        tau[p][p] *= 2.0f / 3.0f * mu;
+    }
+    if (ghost) {
+        float diag = 0.f;
+#pragma unroll
+        for (int p = 0; p < D; p++) diag -= du[p][p];
+        diag *= mu * 2.0f / 3.0f;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) tmc[i][j] = mu * du[j][i];
+            tmc[i][i] += diag;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++)
+                tmc[i][j] = LERP32(w, a.tauMC[(i * D + j) * nt + P], a.tauMC[(i * D + j) * nt + N]);
+    }
+    const float Sd = dot32<D>(S, d);
+    float K[D], dm2 = 0.f;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        const float delta = d[i] * Sm2 / Sd;
+        dm2 += delta * delta;
+        K[i] = S[i] - delta;
+    }
+    const float dm = sqrtf(dm2);
+    float sU[D];
+    if (ghost) {
+        float Uf[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) Uf[i] = LERP32(w, sp.u[i], sn.u[i]);
+#pragma unroll
+        for (int i = 0; i < D; i++) sU[i] = dot32<D>(Uf, tau[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++) sU[i] = LERP32(w, a.sigmaU[i * nt + P], a.sigmaU[i * nt + N]);
+    }
+    const float kcond = a.cp * mu * a.pr_inv;
+#pragma unroll
+    for (int i = 0; i < D; i++)
+        rhs[i + 1] += dot32<D>(tmc[i], S) + mu * (dm * (sn.up[i] - sp.up[i]) * dmi + dot32<D>(K, du[i]));
+    rhs[D + 1] += dot32<D>(sU, S) + kcond * (dm * (aT - cT) * dmi + dot32<D>(K, dT));
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) a.frhs[(size_t)k * a.nf + f] = rhs[k];
+}
+
+// ---------------------------------------------------------------------------
+// ordered gather of face fluxes, sponge, RK update (deviations)
+// ---------------------------------------------------------------------------
+template <int D, bool RES>
+__global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B, int zero_dq) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[256 / 64][D + 2];
+    double rsq[D + 2];
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) rsq[k] = 0.0;
+    if (c < a.nc) {
+        const float dt = (float)(*a.dt);
+        const float vinv = a.vinv[c];
+        float dq[D + 2], res[D + 2];
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            dq[k] = zero_dq ? 0.f : a.dq[(size_t)k * a.nc + c] * A;
+            res[k] = 0.f;
+        }
+        const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
+        for (int e = e0; e < e1; e++) {
+            const int ent = a.g_ent[e];
+            const int f = ent >= 0 ? ent : ~ent;
+            const float sg = ent >= 0 ? 1.f : -1.f;
+#pragma unroll
+            for (int k = 0; k < D + 2; k++) {
+                const float r = sg * a.frhs[(size_t)k * a.nf + f];
+                if (RES) res[k] += r;
+                dq[k] += dt * r * vinv;
+            }
+        }
+        float q[D + 2];
+        load_q32<D>(a, c, q);
+        const float sgm = a.sigma[c];
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            dq[k] -= dt * sgm * q[k];  // sponge: sigma (q_inf - q) = -sigma q'
+            q[k] += B * dq[k];
+            a.q[(size_t)k * a.ntot + c] = q[k];
+            a.dq[(size_t)k * a.nc + c] = dq[k];
+        }
+        if (RES) {
+#pragma unroll
+            for (int k = 0; k < D + 2; k++) {
+                a.res_cell[(size_t)k * a.nc + c] = res[k];
+                rsq[k] = (double)res[k] * (double)res[k];
+            }
+        }
+    }
+    if (RES) {
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            double v = rsq[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0) red[wid][k] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < D + 2; k++) {
+                double s = 0.0;
+                for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i][k];
+                a.partial[(size_t)blockIdx.x * (D + 2) + k] = s;
+            }
+        }
+    }
+}
+
+// compute_dt / compute_cfl in FP64 from the FP32 deviations
+template <int D, int MODE>
+__global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sh[256];
+    double v = MODE == 0 ? 100.0 : 0.0;
+    if (c < a.nc) {
+        const double rho = a.rho_inf_d + (double)a.q[c];
+        double m[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) m[k] = a.m_inf_d[k] + (double)a.q[(size_t)(k + 1) * a.ntot + c];
+        double sum = 0.0;
+        for (int e = a.s_off[c]; e < a.s_off[c + 1]; e++) {
+            const int ent = a.s_ent[e];
+            const int f = ent >= 0 ? ent : ~ent;
+            double S[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                const double s = (double)a.S[(size_t)k * a.nf + f];
+                S[k] = ent >= 0 ? s : -s;
+            }
+            double fl = 0.0;
+            if (MODE == 0) {
+                double sM = 0.0;
+#pragma unroll
+                for (int k = 0; k < D; k++) sM += S[k] * S[k];
+                sM = sqrt(sM);
+#pragma unroll
+                for (int k = 0; k < D; k++) fl += m[k] * S[k] / sM;
+            } else {
+#pragma unroll
+                for (int k = 0; k < D; k++) fl += m[k] * S[k];
+            }
+            sum += fabs(fl);
+        }
+        const double vol = a.vol_d[c];
+        if (MODE == 0) {
+            const double dtc = param * rho * 2.0 * vol / sum;
+            if (v > dtc) v = dtc;
+        } else {
+            const double cfl = sum * ((1.0 / rho) * (1.0 / vol) * param);
+            if (v < cfl) v = cfl;
+        }
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const double o = sh[threadIdx.x + st];
+            if (MODE == 0) {
+                if (sh[threadIdx.x] > o) sh[threadIdx.x] = o;
+            } else {
+                if (sh[threadIdx.x] < o) sh[threadIdx.x] = o;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = sh[0];
+}
+
+// FP64 conservation integrals sum_c V_c q_c (per block partials)
+template <int D>
+__global__ void __launch_bounds__(256) integrals32(KArgs32 a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[256 / 64][D + 2];
+    double v[D + 2];
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) v[k] = 0.0;
+    if (c < a.nc) {
+        const double vol = a.vol_d[c];
+        v[0] = vol * (a.rho_inf_d + (double)a.q[c]);
+#pragma unroll
+        for (int k = 0; k < D; k++) v[k + 1] = vol * (a.m_inf_d[k] + (double)a.q[(size_t)(k + 1) * a.ntot + c]);
+        v[D + 1] = vol * (a.rhoE_inf_d + (double)a.q[(size_t)(D + 1) * a.ntot + c]);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) red[wid][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            double s = 0.0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i][k];
+            a.partial[(size_t)blockIdx.x * (D + 2) + k] = s;
+        }
+    }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) count_nonfinite32(const float *q, int nc, int ntot, unsigned long long *cnt) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int bad = 0;
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) bad += !isfinite(q[(size_t)k * ntot + c]);
+    if (bad) atomicAdd(cnt, (unsigned long long)bad);
+}
+
+// launchers (defined in solver_fp32p.hip)
+template <int D>
+void launch_bc32(Solver *s);
+template <int D>
+void launch_gradvis32(Solver *s);
+template <int D>
+void launch_flux32(Solver *s);
+template <int D>
+void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool accumulate_res);
+template <int D>
+void launch_cfl_dt32(Solver *s, int mode, double param, double *out, double *dt_dst);
 
 }  // namespace lfm

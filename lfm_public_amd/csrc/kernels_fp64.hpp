@@ -719,7 +719,7 @@ __global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) cfl_dt_final(const double *partial, int nblk, double *out, double *dt_dst) {
+static __global__ void __launch_bounds__(256) cfl_dt_final(const double *partial, int nblk, double *out, double *dt_dst) {
     __shared__ double sh[256];
     double v = MODE == 0 ? 100.0 : 0.0;
     for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
@@ -746,6 +746,32 @@ __global__ void __launch_bounds__(256) cfl_dt_final(const double *partial, int n
     if (threadIdx.x == 0) {
         *out = sh[0];
         if (dt_dst) *dt_dst = sh[0];
+    }
+}
+
+// conservation integrals sum_c V_c q_c, per-block partials (FP64)
+template <int D>
+__global__ void __launch_bounds__(256) integrals64(KArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[256 / 64][D + 2];
+    double v[D + 2];
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) v[k] = c < a.nc ? a.q[(size_t)k * a.ntot + c] / a.vinv[c] : 0.0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) red[wid][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            double s = 0.0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i][k];
+            a.partial[(size_t)blockIdx.x * (D + 2) + k] = s;
+        }
     }
 }
 

@@ -64,6 +64,7 @@ static int dev_zeros(Solver *s, T **dst, size_t n) {
 
 Solver::~Solver() {
     for (void *p : allocs) (void)hipFree(p);
+    if (k32) free_k32(k32);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -193,6 +194,16 @@ static void launch_cfl_dt64(Solver *s, int mode, double param, double *out, doub
         hipLaunchKernelGGL((cfl_dt_cells<D, 1>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, param);
         hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kBlock), 0, s->stream, s->k64.partial, nb, out, dt_dst);
     }
+}
+
+int integrals64(Solver *s, double *out_dev) {
+    const int nb = nblocks(s->L.nc);
+    if (s->L.D == 2)
+        hipLaunchKernelGGL(integrals64<2>, dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+    else
+        hipLaunchKernelGGL(integrals64<3>, dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+    hipLaunchKernelGGL(reduce_sum, dim3(1), dim3(kBlock), 0, s->stream, s->k64.partial, nb, s->L.D + 2, out_dev, 0);
+    return 0;
 }
 
 // dispatch helpers over dimension
@@ -529,6 +540,18 @@ int lfm_get_residual(lfm_solver *h, double *res) {
                            s->stream));
     HIP_TRY(hipStreamSynchronize(s->stream));
     for (int k = 0; k < s->L.D + 2; k++) res[k] = s->h_scalar[16 + k];
+    return LFM_OK;
+}
+
+int lfm_conservation(lfm_solver *h, double *out) {
+    if (!h || !out) return fail(LFM_EINVAL, "null argument");
+    Solver *s = h->impl.get();
+    lfm::integrals(s, s->d_scalar + 10);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(s->h_scalar + 24, s->d_scalar + 10, (s->L.D + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < s->L.D + 2; k++) out[k] = s->h_scalar[24 + k];
     return LFM_OK;
 }
 

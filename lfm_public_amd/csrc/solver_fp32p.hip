@@ -1,0 +1,273 @@
+// solver_fp32p.hip -- creation and launches of the FP32-perturbation path.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "kernels_fp32p.hpp"
+#include "solver_state.hpp"
+
+namespace lfm {
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(LFM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+static constexpr int kBlock = 256;
+static inline int nblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+template <typename T>
+static int up(Solver *s, T **dst, const T *src, size_t n) {
+    if (n == 0) {
+        *dst = nullptr;
+        return 0;
+    }
+    HIP_TRY(hipMalloc((void **)dst, n * sizeof(T)));
+    s->allocs.push_back((void *)*dst);
+    s->device_bytes += (int64_t)(n * sizeof(T));
+    if (src)
+        HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    else
+        HIP_TRY(hipMemset(*dst, 0, n * sizeof(T)));
+    return 0;
+}
+
+static std::vector<float> to_f32(const std::vector<double> &v) {
+    std::vector<float> o(v.size());
+    for (size_t i = 0; i < v.size(); i++) o[i] = (float)v[i];
+    return o;
+}
+
+int create_fp32p(Solver *s) {
+    const SolverLayout &L = s->L;
+    const int D = L.D;
+    const size_t nc = L.nc, ntot = (size_t)L.nc + L.ng, nf = L.nf;
+    s->k32 = new KArgs32();
+    KArgs32 &a = *s->k32;
+    std::memset(&a, 0, sizeof(a));
+    a.nc = L.nc;
+    a.nsm0 = L.nsm0;
+    a.ng = L.ng;
+    a.nf = L.nf;
+    a.ntot = (int)ntot;
+    const Gas &g = s->gas;
+    // freestream constants (FP64 derivations, rounded once)
+    double m_inf[3] = {g.rho_inf * g.U_inf[0], g.rho_inf * g.U_inf[1], g.rho_inf * g.U_inf[2]};
+    const double rhoE_inf = g.rho_inf * g.E_inf;
+    double uinf2 = 0.0;
+    for (int k = 0; k < D; k++) uinf2 += g.U_inf[k] * g.U_inf[k];
+    const double Rpsi_inf = g.p_inf / g.rho_inf;
+    const double gog = g.gamma / g.gm1;
+    const double H_inf = gog * Rpsi_inf + 0.5 * uinf2;
+    a.gamma = (float)g.gamma;
+    a.gm1 = (float)g.gm1;
+    a.gog = (float)gog;
+    a.R_inv = (float)g.R_inv;
+    a.mu = (float)g.mu;
+    a.cp = (float)g.cp;
+    a.pr_inv = (float)g.pr_inv;
+    a.rho_inf = (float)g.rho_inf;
+    a.p_inf = (float)g.p_inf;
+    a.T_inf = (float)g.T_inf;
+    a.Rpsi_inf = (float)Rpsi_inf;
+    a.E_inf = (float)g.E_inf;
+    a.H_inf = (float)H_inf;
+    a.G_inf = (float)(g.rho_inf * H_inf);
+    a.c_inf = (float)std::sqrt(g.gamma * Rpsi_inf);
+    for (int k = 0; k < 3; k++) {
+        a.uinf[k] = k < D ? (float)g.U_inf[k] : 0.f;
+        a.minf[k] = k < D ? (float)m_inf[k] : 0.f;
+        a.m_inf_d[k] = k < D ? m_inf[k] : 0.0;
+    }
+    a.rho_inf_d = g.rho_inf;
+    a.rhoE_inf_d = rhoE_inf;
+
+    int rc = 0;
+    rc |= up(s, (int32_t **)&a.fP, L.fP.data(), nf);
+    rc |= up(s, (int32_t **)&a.fN, L.fN.data(), nf);
+    {
+        auto S = to_f32(L.S), d = to_f32(L.d), w = to_f32(L.w);
+        rc |= up(s, (float **)&a.S, S.data(), S.size());
+        rc |= up(s, (float **)&a.d, d.data(), d.size());
+        rc |= up(s, (float **)&a.w, w.data(), w.size());
+        auto vinv = to_f32(L.vinv), sig = to_f32(L.sigma);
+        rc |= up(s, (float **)&a.vinv, vinv.data(), nc);
+        rc |= up(s, (float **)&a.sigma, sig.data(), nc);
+    }
+    rc |= up(s, (int32_t **)&a.g_off, L.g_off.data(), nc + 1);
+    rc |= up(s, (int32_t **)&a.g_ent, L.g_ent.data(), L.g_ent.size());
+    rc |= up(s, (int32_t **)&a.s_off, L.s_off.data(), nc + 1);
+    rc |= up(s, (int32_t **)&a.s_ent, L.s_ent.data(), L.s_ent.size());
+    rc |= up(s, (int32_t **)&a.ghost_cell, L.ghost_cell.data(), L.ng);
+    rc |= up(s, (int32_t **)&a.ghost_role, L.ghost_role.data(), L.ng);
+    rc |= up(s, (double **)&a.vol_d, L.volume.data(), nc);
+    // initial deviations; unset-patch ghosts hold q = 0 (q' = -q_inf)
+    {
+        std::vector<float> q0((size_t)(D + 2) * ntot, 0.f);
+        const double qinf[5] = {g.rho_inf, m_inf[0], m_inf[1], m_inf[2], rhoE_inf};
+        double qi[5];
+        qi[0] = qinf[0];
+        for (int k = 0; k < D; k++) qi[k + 1] = qinf[k + 1];
+        qi[D + 1] = rhoE_inf;
+        for (int k = 0; k < D + 2; k++) {
+            for (size_t c = 0; c < nc; c++) q0[k * ntot + c] = (float)(L.q0[k * nc + c] - qi[k]);
+            for (size_t gi = 0; gi < (size_t)L.ng; gi++) q0[k * ntot + nc + gi] = (float)(-qi[k]);
+        }
+        rc |= up(s, &a.q, q0.data(), q0.size());
+    }
+    rc |= up(s, &a.dq, (float *)nullptr, (size_t)(D + 2) * nc);
+    rc |= up(s, &a.res_cell, (float *)nullptr, (size_t)(D + 2) * nc);
+    rc |= up(s, &a.dudx, (float *)nullptr, (size_t)D * D * ntot);
+    rc |= up(s, &a.dTdx, (float *)nullptr, (size_t)D * ntot);
+    rc |= up(s, &a.tauMC, (float *)nullptr, (size_t)D * D * ntot);
+    rc |= up(s, &a.sigmaU, (float *)nullptr, (size_t)D * ntot);
+    if (s->prm.solver == 2) {
+        rc |= up(s, &a.rho_grad, (float *)nullptr, (size_t)D * ntot);
+        rc |= up(s, &a.p_grad, (float *)nullptr, (size_t)D * ntot);
+        rc |= up(s, &a.U_grad, (float *)nullptr, (size_t)D * D * ntot);
+    }
+    rc |= up(s, &a.frhs, (float *)nullptr, (size_t)(D + 2) * nf);
+    const size_t npart = (size_t)nblocks((long)std::max(nc, nf)) * (D + 2) + 16;
+    rc |= up(s, &a.partial, (double *)nullptr, npart);
+    rc |= up(s, &s->d_scalar, (double *)nullptr, 16);
+    if (rc) return LFM_EDEVICE;
+    a.dt = s->d_scalar;
+    s->d_res = s->d_scalar + 2;
+    s->d_cfl = s->d_scalar + 8;
+    s->d_dtmin = s->d_scalar + 9;
+    s->k64.partial = a.partial;  // shared reduction scratch for the final kernels
+    HIP_TRY(hipMemcpy(s->d_scalar, &s->dt, sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <int D>
+void launch_bc32(Solver *s) {
+    if (s->L.ng) hipLaunchKernelGGL(bc_fill32<D>, dim3(nblocks(s->L.ng)), dim3(kBlock), 0, s->stream, *s->k32);
+}
+
+template <int D>
+void launch_gradvis32(Solver *s) {
+    const int nb = nblocks(s->L.nc);
+    if (s->prm.solver == 2 && s->prm.minmod)
+        hipLaunchKernelGGL((grad_vis32<D, 2>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+    else
+        hipLaunchKernelGGL((grad_vis32<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+}
+
+template <int D>
+void launch_flux32(Solver *s) {
+    const int nb = nblocks(s->L.nf);
+    switch (s->prm.solver) {
+    case 0: hipLaunchKernelGGL((face_flux32<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
+    case 1: hipLaunchKernelGGL((face_flux32<D, 1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
+    default: hipLaunchKernelGGL((face_flux32<D, 2>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
+    }
+}
+
+template <int D>
+void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool accumulate_res) {
+    const int nb = nblocks(s->L.nc);
+    if (res) {
+        hipLaunchKernelGGL((cell_update32<D, true>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
+                           (float)B, (int)zero_dq);
+        hipLaunchKernelGGL(reduce_sum, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, D + 2, s->d_res,
+                           (int)accumulate_res);
+    } else {
+        hipLaunchKernelGGL((cell_update32<D, false>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
+                           (float)B, (int)zero_dq);
+    }
+}
+
+template <int D>
+void launch_cfl_dt32(Solver *s, int mode, double param, double *out, double *dt_dst) {
+    const int nb = nblocks(s->L.nc);
+    if (mode == 0) {
+        hipLaunchKernelGGL((cfl_dt_cells32<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, param);
+        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, out, dt_dst);
+    } else {
+        hipLaunchKernelGGL((cfl_dt_cells32<D, 1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, param);
+        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, out, dt_dst);
+    }
+}
+
+template <int D>
+void launch_integrals32(Solver *s, double *out) {
+    const int nb = nblocks(s->L.nc);
+    hipLaunchKernelGGL(integrals32<D>, dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+    hipLaunchKernelGGL(reduce_sum, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, D + 2, out, 0);
+}
+
+template <int D>
+void launch_nonfinite32(Solver *s, unsigned long long *d) {
+    hipLaunchKernelGGL(count_nonfinite32<D>, dim3(nblocks(s->L.nc)), dim3(kBlock), 0, s->stream, s->k32->q, s->L.nc,
+                       s->L.nc + s->L.ng, d);
+}
+
+// FP32 field access (converted to FP64 values in original cell order)
+int get_field32(Solver *s, int which, double *out) {
+    const SolverLayout &L = s->L;
+    const int D = L.D;
+    const size_t nc = L.nc, nt = (size_t)L.nc + L.ng;
+    KArgs32 &a = *s->k32;
+    const float *ptr = nullptr;
+    int ncomp = 0;
+    size_t stride = nt;
+    switch (which) {
+    case LFM_FIELD_Q: ptr = a.q; ncomp = D + 2; break;
+    case LFM_FIELD_DQ: ptr = a.dq; ncomp = D + 2; stride = nc; break;
+    case LFM_FIELD_DUDX: ptr = a.dudx; ncomp = D * D; break;
+    case LFM_FIELD_DTDX: ptr = a.dTdx; ncomp = D; break;
+    case LFM_FIELD_TAUMC: ptr = a.tauMC; ncomp = D * D; break;
+    case LFM_FIELD_SIGMAU: ptr = a.sigmaU; ncomp = D; break;
+    case LFM_FIELD_RHOGRAD: ptr = a.rho_grad; ncomp = D; break;
+    case LFM_FIELD_PGRAD: ptr = a.p_grad; ncomp = D; break;
+    case LFM_FIELD_UGRAD: ptr = a.U_grad; ncomp = D * D; break;
+    case LFM_FIELD_RES: ptr = a.res_cell; ncomp = D + 2; stride = nc; break;
+    default: return fail(LFM_EINVAL, "unknown field");
+    }
+    if (!ptr) return fail(LFM_EINVAL, "field not allocated for this solver");
+    double qinf[5] = {a.rho_inf_d, a.m_inf_d[0], a.m_inf_d[1], a.m_inf_d[2], 0.0};
+    qinf[D + 1] = a.rhoE_inf_d;
+    std::vector<float> tmp(nc);
+    for (int k = 0; k < ncomp; k++) {
+        HIP_TRY(hipMemcpy(tmp.data(), ptr + k * stride, nc * sizeof(float), hipMemcpyDeviceToHost));
+        const double base = which == LFM_FIELD_Q ? qinf[k] : 0.0;
+        double *o = out + k * nc;
+        for (size_t ci = 0; ci < nc; ci++) o[L.orig[ci]] = base + (double)tmp[ci];
+    }
+    return LFM_OK;
+}
+
+void free_k32(KArgs32 *k) { delete k; }
+
+int integrals(Solver *s, double *out_dev) {
+    if (!s->fp32) return integrals64(s, out_dev);
+    if (s->L.D == 2)
+        launch_integrals32<2>(s, out_dev);
+    else
+        launch_integrals32<3>(s, out_dev);
+    return 0;
+}
+
+int nonfinite32(Solver *s, unsigned long long *d) {
+    if (s->L.D == 2) launch_nonfinite32<2>(s, d);
+    else launch_nonfinite32<3>(s, d);
+    return 0;
+}
+
+template void launch_bc32<2>(Solver *);
+template void launch_bc32<3>(Solver *);
+template void launch_gradvis32<2>(Solver *);
+template void launch_gradvis32<3>(Solver *);
+template void launch_flux32<2>(Solver *);
+template void launch_flux32<3>(Solver *);
+template void launch_update32<2>(Solver *, double, double, bool, bool, bool);
+template void launch_update32<3>(Solver *, double, double, bool, bool, bool);
+template void launch_cfl_dt32<2>(Solver *, int, double, double *, double *);
+template void launch_cfl_dt32<3>(Solver *, int, double, double *, double *);
+
+}  // namespace lfm

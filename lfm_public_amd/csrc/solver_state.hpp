@@ -71,5 +71,11 @@ struct Solver {
 };
 
 int create_fp32p(Solver *s);
+void free_k32(KArgs32 *k);
+int get_field32(Solver *s, int which, double *out);
+int nonfinite32(Solver *s, unsigned long long *d);
+// FP64 conservation integrals sum_c V_c q_c into out_dev[0..D+1] (device)
+int integrals(Solver *s, double *out_dev);
+int integrals64(Solver *s, double *out_dev);
 
 }  // namespace lfm

@@ -840,7 +840,7 @@ static void viscous_face(const oracle *o, const cellgeo *G, int f, const solvars
     const double *S = G->S[f], *d = G->d[f];
     double d_mag = magD(D, d);
     double dmi = 1.0 / d_mag;
-    double dn[3], dudx[3][3], dTdx[3], tau[3][3], tauMC[3][3], divT[3], delta[3], K[3], lapU[3], sU[3];
+    double dn[3], dudx[3][3], dTdx[3], tau[3][3], tauMC[3][3], divT[3], delta[3], K[3], lapU[3], sU[3] = {0, 0, 0};
     for (int k = 0; k < D; k++) dn[k] = d[k] * dmi;
     for (int i = 0; i < D; i++) {
         for (int j = 0; j < D; j++) dudx[i][j] = INTERP_LINEAR(w, cell->dudx[i][j], nb->dudx[i][j]);
@@ -1228,5 +1228,70 @@ int lfmo_get_slots(void *h, int *nface, int *nvalid, int *nbr, double *S, double
             }
         }
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* unit-level entry points for the oracle's own closed-form tests      */
+/* ------------------------------------------------------------------ */
+double lfmo_minmod(int D, double cphi, double aphi, const double *grad, const double *d, double wl, double flux)
+{
+    return interp_minmod(D, cphi, aphi, grad, d, wl, flux);
+}
+
+void lfmo_p5(double M, double alpha, double *pos_neg)
+{
+    pos_neg[0] = p5Pos(M, alpha);
+    pos_neg[1] = p5Neg(M, alpha);
+}
+
+/* rhs of one face (convective + viscous) as one_rk_step_* computes it, for
+ * given cell states; vis/grad arrays hold dudx[D*D], dTdx[D], tauMC[D*D],
+ * sigmaU[D], rho_grad[D], p_grad[D], U_grad[D*D] for P then N. */
+int lfmo_face_rhs(const lfmo_params *prm, const double *qP, const double *qN, const double *S, const double *d,
+                  double w, int ghost, const double *visP, const double *visN, double *rhs)
+{
+    oracle o;
+    memset(&o, 0, sizeof o);
+    const int D = prm->dim;
+    o.D = D;
+    o.solver = prm->solver;
+    o.mu = prm->mu;
+    o.cp = prm->cp;
+    o.R = 8.31447 / prm->mol_weight * 1000;
+    o.gamma = o.cp / (o.cp - o.R);
+    o.gm1 = o.gamma - 1.0;
+    o.R_inv = 1.0 / o.R;
+    o.pr_inv = 1.0 / prm->pr;
+    solvars c, n;
+    memset(&c, 0, sizeof c);
+    memset(&n, 0, sizeof n);
+    cellgeo G;
+    memset(&G, 0, sizeof G);
+    for (int k = 0; k < D + 2; k++) { c.q[k] = qP[k]; n.q[k] = qN[k]; }
+    for (int k = 0; k < D; k++) { G.S[0][k] = S[k]; G.d[0][k] = d[k]; }
+    G.w[0] = w;
+    n.is_ghost = ghost;
+    solvars *cells[2] = {&c, &n};
+    const double *vis[2] = {visP, visN};
+    for (int s = 0; s < 2; s++) {
+        const double *v = vis[s];
+        solvars *x = cells[s];
+        int o_ = 0;
+        for (int i = 0; i < D; i++) for (int j = 0; j < D; j++) x->dudx[i][j] = v[o_++];
+        for (int i = 0; i < D; i++) x->dTdx[i] = v[o_++];
+        for (int i = 0; i < D; i++) for (int j = 0; j < D; j++) x->tauMC[i][j] = v[o_++];
+        for (int i = 0; i < D; i++) x->sigmaU[i] = v[o_++];
+        for (int i = 0; i < D; i++) x->rho_grad[i] = v[o_++];
+        for (int i = 0; i < D; i++) x->p_grad[i] = v[o_++];
+        for (int i = 0; i < D; i++) for (int j = 0; j < D; j++) x->U_grad[i][j] = v[o_++];
+    }
+    double cri = 1.0 / c.q[0], ari = 1.0 / n.q[0], cU[3], aU[3];
+    for (int k = 0; k < D; k++) { cU[k] = c.q[k + 1] * cri; aU[k] = n.q[k + 1] * ari; }
+    double cR = compute_Rpsi(&o, c.q), aR = compute_Rpsi(&o, n.q);
+    double Sm;
+    if (o.solver == 0) convective_M1(&o, &G, 0, &c, &n, cri, ari, cU, aU, cR, aR, &Sm, rhs);
+    else convective_M2(&o, &G, 0, &c, &n, cri, ari, cU, aU, cR, aR, &Sm, rhs, o.solver == 2);
+    viscous_face(&o, &G, 0, &c, &n, cU, aU, cR * o.R_inv, aR * o.R_inv, Sm, rhs);
     return 0;
 }

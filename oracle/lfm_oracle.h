@@ -101,6 +101,12 @@ int    lfmo_get_slots(void *h, int *nface, int *nvalid, int *nbr,
                       double *S, double *d, double *w);
 int    lfmo_n_ghost(void *h);
 
+/* Unit-level pieces (closed-form tests of the oracle itself). */
+double lfmo_minmod(int D, double cphi, double aphi, const double *grad, const double *d, double wl, double flux);
+void   lfmo_p5(double M, double alpha, double *pos_neg);
+int    lfmo_face_rhs(const lfmo_params *prm, const double *qP, const double *qN, const double *S, const double *d,
+                     double w, int ghost, const double *visP, const double *visN, double *rhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,12 +70,53 @@ def lib():
         L.lfmo_compute_cfl.argtypes = [C.c_void_p, C.c_double]
         L.lfmo_get_slots.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         L.lfmo_n_ghost.argtypes = [C.c_void_p]
+        L.lfmo_minmod.restype = C.c_double
+        L.lfmo_minmod.argtypes = [C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+        L.lfmo_p5.argtypes = [C.c_double, C.c_double, C.c_void_p]
+        L.lfmo_face_rhs.argtypes = [C.POINTER(_Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                    C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
 
 def _ptr(a):
     return a.ctypes.data if a is not None else None
+
+
+def _params(params: dict) -> _Params:
+    pr = _Params()
+    for k, v in params.items():
+        if k == "U_inf":
+            for i in range(3):
+                pr.U_inf[i] = float(v[i])
+        elif hasattr(pr, k):
+            setattr(pr, k, v)
+    return pr
+
+
+def minmod(D, cphi, aphi, grad, d, wl, flux) -> float:
+    g = np.ascontiguousarray(grad, np.float64)
+    dd = np.ascontiguousarray(d, np.float64)
+    return lib().lfmo_minmod(D, cphi, aphi, _ptr(g), _ptr(dd), wl, flux)
+
+
+def p5(M, alpha):
+    out = np.zeros(2)
+    lib().lfmo_p5(M, alpha, _ptr(out))
+    return out[0], out[1]
+
+
+def face_rhs(params: dict, qP, qN, S, d, w, ghost=False, visP=None, visN=None) -> np.ndarray:
+    D = params["dim"]
+    nvis = 3 * D * D + 4 * D
+    vp = np.zeros(nvis) if visP is None else np.ascontiguousarray(visP, np.float64)
+    vn = np.zeros(nvis) if visN is None else np.ascontiguousarray(visN, np.float64)
+    a = [np.ascontiguousarray(x, np.float64) for x in (qP, qN, S, d)]
+    rhs = np.zeros(D + 2)
+    pr = _params(params)
+    lib().lfmo_face_rhs(C.byref(pr), _ptr(a[0]), _ptr(a[1]), _ptr(a[2]), _ptr(a[3]), w, int(ghost), _ptr(vp),
+                        _ptr(vn), _ptr(rhs))
+    return rhs
 
 
 class Oracle:

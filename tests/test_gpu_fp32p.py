@@ -1,0 +1,100 @@
+"""FP32 deviation-from-freestream path vs the FP64 path and the oracle
+(needs an MI355X).
+
+Bar: perturbation-normalised relative L2 error <= 1e-5 on p, U and T
+against FP64 (||x - x64|| / ||x64 - x_inf||), the tolerance the port's
+specification sets for the FP32 path.
+"""
+import numpy as np
+import pytest
+
+from conftest import pert_l2, primitives
+import small_cases as sc
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _errors(case, q32, q64):
+    from lfm_public_amd import cases
+    g, R = cases.gamma_of(case.params), cases.gas_constant(case.params["mol_weight"])
+    D = case.params["dim"]
+    p3, U3, T3 = primitives(q32, D, g, R)
+    p6, U6, T6 = primitives(q64, D, g, R)
+    errs = {"p": pert_l2(p3, p6, case.params["p_inf"]), "T": pert_l2(T3, T6, case.params["T_inf"])}
+    for k in range(D):
+        errs[f"U{k}"] = pert_l2(U3[k], U6[k], case.params["U_inf"][k])
+    return errs
+
+
+@pytest.mark.parametrize("name", sorted(sc.PARITY_CASES))
+def test_fp32p_matches_fp64(name):
+    from lfm_public_amd import capi
+    print(f"[fp32p] {name}", flush=True)
+    case = sc.PARITY_CASES[name]()
+    g64 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP64)
+    g32 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP32P)
+    n = 20
+    g64.run(n)
+    g32.run(n)
+    q64, q32 = g64.q(), g32.q()
+    assert np.all(np.isfinite(q32))
+    errs = _errors(case, q32, q64)
+    print(name, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= TOL, (k, v)
+    # dt / CFL reductions are FP64 on both paths
+    i64, i32 = g64.info(), g32.info()
+    if case.params.get("adjust_dt"):
+        assert i32.dt == pytest.approx(i64.dt, rel=1e-5)
+    else:
+        assert i32.cfl == pytest.approx(i64.cfl, rel=1e-5)
+
+
+@pytest.mark.parametrize("name", ["vortex_M1_euler", "vortex_M2AUSM_minmod", "cylinder_M2AUSM_sponge"])
+def test_fp32p_long_run(name):
+    """200 RK5 steps: the error stays at the 1e-5 bar as it accumulates."""
+    from lfm_public_amd import capi
+    case = sc.PARITY_CASES[name]()
+    g64 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP64)
+    g32 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP32P)
+    g64.run(200)
+    g32.run(200)
+    errs = _errors(case, g32.q(), g64.q())
+    print(name, "200 steps", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= TOL, (k, v)
+
+
+def test_fp32p_freestream_exact():
+    """Uniform freestream: every deviation is exactly zero and stays so."""
+    from lfm_public_amd import capi
+    case = sc.with_params(sc.box(6, solver=2, mu=1e-3), minmod=1)
+    case.p[:] = case.params["p_inf"]
+    case.T[:] = case.params["T_inf"]
+    g = capi.GpuSolver.from_case(case, precision=capi.PREC_FP32P)
+    q0 = g.q()
+    g.run(10)
+    assert np.array_equal(g.q(), q0)
+
+
+def test_fp32p_conservation_closed_box():
+    """All-wall box (closed): mass and energy integrals (FP64 sums over the
+    FP32 state) are conserved to FP32 round-off of the deviations."""
+    from lfm_public_amd import capi, meshgen
+    case = sc.seeded_perturbation(sc.box(8, solver=1, mu=1e-3), amp=1e-3)
+    roles = [meshgen.BC_WALL] * 6
+    g = capi.GpuSolver.from_case(case, precision=capi.PREC_FP32P, roles=roles)
+    c0 = g.conservation()
+    g.run(50)
+    c1 = g.conservation()
+    # deviation-scale tolerance: mass and energy fluxes through walls vanish
+    assert abs(c1[0] - c0[0]) <= 1e-6 * abs(c0[0])
+    assert abs(c1[-1] - c0[-1]) <= 1e-6 * abs(c0[-1])
+    g64 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP64, roles=roles)
+    d0 = g64.conservation()
+    g64.run(50)
+    d1 = g64.conservation()
+    assert abs(d1[0] - d0[0]) <= 1e-13 * abs(d0[0])
+    assert abs(d1[-1] - d0[-1]) <= 1e-13 * abs(d0[-1])
