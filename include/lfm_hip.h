@@ -161,6 +161,24 @@ int lfm_conservation(lfm_solver *s, double *out);
 /* Number of non-finite conservatives in the state (0 = healthy). */
 int lfm_check_finite(lfm_solver *s, int64_t *n_bad);
 
+/* ---------------------------------------------------------------------
+ * Shared launches over several resident solvers (no counterpart in the
+ * reference, which runs one solve per process): the FP32-perturbation
+ * solvers of one scheme class (dimension, solver, minmod, rkOrder; constant
+ * dt, no residual output) are stepped by single kernel launches covering all
+ * of them, one RK step captured as a hipGraph and replayed.  Each solver
+ * computes exactly what lfm_run would compute for it alone.  While a solver
+ * is in a batch, only the batch may step it.
+ * --------------------------------------------------------------------- */
+typedef struct lfm_batch lfm_batch;
+int lfm_batch_create(lfm_batch **out);
+int lfm_batch_destroy(lfm_batch *b);
+/* Replace the member set (may be empty) and capture the step graph. */
+int lfm_batch_set(lfm_batch *b, lfm_solver *const *solvers, int n);
+/* Queue n steps of every member on the batch's stream. */
+int lfm_batch_run(lfm_batch *b, int nsteps);
+int lfm_batch_synchronize(lfm_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,8 @@ EXPORTS = ("lfm_patch_role", "lfm_solver_create", "lfm_solver_destroy", "lfm_las
            "lfm_prepare_for_timestep", "lfm_prepare_for_rkstep", "lfm_set_boundary_conditions",
            "lfm_calc_gradients", "lfm_calc_vis", "lfm_one_rk_step", "lfm_compute_dt", "lfm_compute_cfl", "lfm_run",
            "lfm_synchronize", "lfm_time_steps", "lfm_get_field", "lfm_set_field", "lfm_get_residual",
-           "lfm_check_finite", "lfm_profile_steps", "lfm_conservation")
+           "lfm_check_finite", "lfm_profile_steps", "lfm_conservation", "lfm_batch_create", "lfm_batch_destroy",
+           "lfm_batch_set", "lfm_batch_run", "lfm_batch_synchronize")
 
 
 class LfmError(RuntimeError):
@@ -92,6 +93,11 @@ def load() -> C.CDLL:
     L.lfm_get_residual.argtypes = [vp, vp]
     L.lfm_check_finite.argtypes = [vp, C.POINTER(C.c_int64)]
     L.lfm_conservation.argtypes = [vp, vp]
+    L.lfm_batch_create.argtypes = [C.POINTER(vp)]
+    L.lfm_batch_destroy.argtypes = [vp]
+    L.lfm_batch_set.argtypes = [vp, vp, C.c_int]
+    L.lfm_batch_run.argtypes = [vp, C.c_int]
+    L.lfm_batch_synchronize.argtypes = [vp]
     _lib = L
     return L
 
@@ -267,3 +273,35 @@ class GpuSolver:
         n = C.c_int64()
         _check(load().lfm_check_finite(self.h, C.byref(n)))
         return n.value
+
+
+class Batch:
+    """Shared launches over resident FP32-perturbation solvers (lfm_batch_*)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(load().lfm_batch_create(C.byref(h)))
+        self.h = h.value
+        self.members = []
+
+    def set(self, solvers):
+        arr = (C.c_void_p * max(1, len(solvers)))(*[s.h for s in solvers])
+        _check(load().lfm_batch_set(self.h, arr, len(solvers)))
+        self.members = list(solvers)
+
+    def run(self, nsteps: int):
+        _check(load().lfm_batch_run(self.h, nsteps))
+
+    def synchronize(self):
+        _check(load().lfm_batch_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().lfm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -65,11 +65,12 @@ class Case:
 
 
 def isentropic_vortex(nx=256, ny=None, L=10.0, strength=5.0, radius=1.0, mach=0.5, seed=101, solver=0,
-                      steps_cfl=0.5, rk_order=5, minmod=0, mu=0.0, thickness=1.0) -> Case:
+                      steps_cfl=0.5, rk_order=5, minmod=0, mu=0.0, thickness=1.0, mesh=None) -> Case:
     """Shu isentropic vortex convected at ``mach`` in a [-L, L]^2 box of air
     (p = 1e5 Pa, T = 300 K, gamma = 1.4), constant dt at CFL ``steps_cfl``."""
     ny = ny or nx
-    mesh = meshgen.cartesian_2d(nx, ny, -L, L, -L, L, thickness=thickness)
+    if mesh is None:
+        mesh = meshgen.cartesian_2d(nx, ny, -L, L, -L, L, thickness=thickness)
     prm = base_params(dim=2, solver=solver, rk_order=rk_order, minmod=minmod, mu=mu)
     g = gamma_of(prm)
     R = gas_constant(prm["mol_weight"])
@@ -98,13 +99,14 @@ def isentropic_vortex(nx=256, ny=None, L=10.0, strength=5.0, radius=1.0, mach=0.
 
 
 def cylinder_2d(n_theta=512, n_quad=None, n_tri=None, mach=0.3, noise=1e-4, seed=202, solver=2, minmod=1,
-                Ls_diam=5.0, Re=150.0, r_out=None, jitter=0.1, rk_order=5, cfl=0.5) -> Case:
+                Ls_diam=5.0, Re=150.0, r_out=None, jitter=0.1, rk_order=5, cfl=0.5, mesh=None) -> Case:
     """Unstructured annulus (quads next to the wall, triangles outside) around a
     unit-diameter cylinder, uniform inflow at ``mach`` with seeded pressure noise,
     sponge of length ``Ls_diam`` diameters measured from the outlet patch."""
     n_quad = n_quad if n_quad is not None else n_theta // 8
     n_tri = n_tri if n_tri is not None else n_theta // 8
-    mesh = meshgen.annulus_mixed(n_theta, n_quad, n_tri, r_in=0.5, r_out=r_out, jitter=jitter, seed=seed)
+    if mesh is None:
+        mesh = meshgen.annulus_mixed(n_theta, n_quad, n_tri, r_in=0.5, r_out=r_out, jitter=jitter, seed=seed)
     prm = base_params(dim=2, solver=solver, rk_order=rk_order, minmod=minmod, p_inf=1.0, T_inf=1.0,
                       mol_weight=NORM_GAS["mol_weight"], cp=NORM_GAS["cp"], pr=NORM_GAS["pr"])
     c_inf = sound_speed(prm)

@@ -17,7 +17,7 @@ namespace lfm {
     } while (0)
 
 // ISolver::prepare_for_timestep: dq = 0, RES = 0
-__global__ void __launch_bounds__(256) fill_zero32(float *p, size_t n) {
+static __global__ void __launch_bounds__(256) fill_zero32(float *p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0f;
 }

@@ -38,6 +38,8 @@ struct KArgs32 {
     const double *dt;
     double *partial;
     const double *vol_d;  // FP64 cell volumes (conservation integrals)
+    double *d_cfl, *d_dtmin;  // this solver's FP64 CFL / dt-min scalars
+    int nblk_cells;           // blocks of 256 over the cells (reduction partials)
     float gamma, gm1, gog, R_inv, mu, cp, pr_inv;
     float rho_inf, p_inf, T_inf, Rpsi_inf, E_inf, H_inf, G_inf, c_inf;
     float uinf[3], minf[3];
@@ -126,8 +128,7 @@ __device__ __forceinline__ float p5neg32(float M, float alpha) {
 // ghost states (cfd_v0.cpp:1010-1133) in deviation form
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bc_fill32_body(const KArgs32 &a, const int g) {
     if (g >= a.ng) return;
     const int c = a.ghost_cell[g];
     const int role = a.ghost_role[g];
@@ -176,8 +177,7 @@ __global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
 // Green-Gauss of deviations + viscous tensors, per cell
 // ---------------------------------------------------------------------------
 template <int D, int GRAD>
-__global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
     if (c >= a.nc) return;
     const float vinv = a.vinv[c];
     float dudx[D][D], dTdx[D], rg[D], pg[D], ug[D][D];
@@ -273,8 +273,7 @@ __global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
 // face fluxes minus the freestream flux, one thread per face
 // ---------------------------------------------------------------------------
 template <int D, int SOLVER>
-__global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) {
     if (f >= a.nf) return;
     const int P = a.fP[f], N = a.fN[f];
     const bool ghost = N >= a.nc;
@@ -524,9 +523,10 @@ __global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
 // ---------------------------------------------------------------------------
 // ordered gather of face fluxes, sponge, RK update (deviations)
 // ---------------------------------------------------------------------------
+// every thread of the block must call this (block-wide reduction when RES)
 template <int D, bool RES>
-__global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B, int zero_dq) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cell_update32_body(const KArgs32 &a, float A, float B, int zero_dq, const int c,
+                                                   const int lblk) {
     __shared__ double red[256 / 64][D + 2];
     double rsq[D + 2];
 #pragma unroll
@@ -584,16 +584,16 @@ __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B
             for (int k = 0; k < D + 2; k++) {
                 double s = 0.0;
                 for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i][k];
-                a.partial[(size_t)blockIdx.x * (D + 2) + k] = s;
+                a.partial[(size_t)lblk * (D + 2) + k] = s;
             }
         }
     }
 }
 
-// compute_dt / compute_cfl in FP64 from the FP32 deviations
+// compute_dt / compute_cfl in FP64 from the FP32 deviations (every thread of
+// the block must call this)
 template <int D, int MODE>
-__global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cfl_dt_cells32_body(const KArgs32 &a, double param, const int c, const int lblk) {
     __shared__ double sh[256];
     double v = MODE == 0 ? 100.0 : 0.0;
     if (c < a.nc) {
@@ -647,7 +647,7 @@ __global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) a.partial[lblk] = sh[0];
 }
 
 // FP64 conservation integrals sum_c V_c q_c (per block partials)
@@ -691,6 +691,97 @@ __global__ void __launch_bounds__(256) count_nonfinite32(const float *q, int nc,
 #pragma unroll
     for (int k = 0; k < D + 2; k++) bad += !isfinite(q[(size_t)k * ntot + c]);
     if (bad) atomicAdd(cnt, (unsigned long long)bad);
+}
+
+// ---------------------------------------------------------------------------
+// single-solver kernels
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
+    bc_fill32_body<D>(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+template <int D, int GRAD>
+__global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
+    grad_vis32_body<D, GRAD>(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+template <int D, int SOLVER>
+__global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
+    face_flux32_body<D, SOLVER>(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+template <int D, bool RES>
+__global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B, int zero_dq) {
+    cell_update32_body<D, RES>(a, A, B, zero_dq, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.x);
+}
+template <int D, int MODE>
+__global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
+    cfl_dt_cells32_body<D, MODE>(a, param, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// batched kernels: one launch covers several resident solvers of one scheme
+// class.  Block b belongs to solver bmap[b] (sorted) and is its local block
+// b - bfirst[solver]; each solver computes exactly what its single launch
+// computes.
+// ---------------------------------------------------------------------------
+struct BatchMap {
+    const int32_t *req;    // [nblocks] solver of each block
+    const int32_t *first;  // [nsolvers] first block of each solver
+};
+
+__device__ __forceinline__ void batch_slot(const BatchMap &m, int &r, int &lblk) {
+    r = m.req[blockIdx.x];
+    lblk = (int)blockIdx.x - m.first[r];
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) bc_fill32_b(const KArgs32 *args, BatchMap m) {
+    int r, lb;
+    batch_slot(m, r, lb);
+    bc_fill32_body<D>(args[r], lb * 256 + threadIdx.x);
+}
+template <int D, int GRAD>
+__global__ void __launch_bounds__(256) grad_vis32_b(const KArgs32 *args, BatchMap m) {
+    int r, lb;
+    batch_slot(m, r, lb);
+    grad_vis32_body<D, GRAD>(args[r], lb * 256 + threadIdx.x);
+}
+template <int D, int SOLVER>
+__global__ void __launch_bounds__(256) face_flux32_b(const KArgs32 *args, BatchMap m) {
+    int r, lb;
+    batch_slot(m, r, lb);
+    face_flux32_body<D, SOLVER>(args[r], lb * 256 + threadIdx.x);
+}
+template <int D>
+__global__ void __launch_bounds__(256) cell_update32_b(const KArgs32 *args, BatchMap m, float A, float B,
+                                                       int zero_dq) {
+    int r, lb;
+    batch_slot(m, r, lb);
+    cell_update32_body<D, false>(args[r], A, B, zero_dq, lb * 256 + threadIdx.x, lb);
+}
+// compute_cfl with each solver's own deltaT
+template <int D>
+__global__ void __launch_bounds__(256) cfl_cells32_b(const KArgs32 *args, BatchMap m) {
+    int r, lb;
+    batch_slot(m, r, lb);
+    const KArgs32 &a = args[r];
+    cfl_dt_cells32_body<D, 1>(a, *a.dt, lb * 256 + threadIdx.x, lb);
+}
+// one block per solver: max of its partials into its CFL scalar
+static __global__ void __launch_bounds__(256) cfl_final32_b(const KArgs32 *args) {
+    __shared__ double sh[256];
+    const KArgs32 &a = args[blockIdx.x];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < a.nblk_cells; i += blockDim.x) {
+        const double o = a.partial[i];
+        if (v < o) v = o;
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st && sh[threadIdx.x] < sh[threadIdx.x + st]) sh[threadIdx.x] = sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *a.d_cfl = sh[0];
 }
 
 // launchers (defined in solver_fp32p.hip)

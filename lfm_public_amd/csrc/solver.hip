@@ -26,6 +26,11 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+std::mutex &device_setup_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 #define HIP_TRY(expr)                                                                            \
     do {                                                                                         \
         hipError_t e_ = (expr);                                                                  \
@@ -303,9 +308,6 @@ void Solver::step() {
 using lfm::fail;
 using lfm::Solver;
 
-struct lfm_solver {
-    std::unique_ptr<Solver> impl;
-};
 
 extern "C" {
 
@@ -342,6 +344,7 @@ int lfm_solver_create(const lfm_polymesh *mesh, const lfm_params *prm, const dou
     s->nstages = lfm::rk_tables(prm->rk_order, s->A, s->B);
     s->dt = prm->dt;
     s->fp32 = prm->precision == LFM_PREC_FP32P;
+    std::lock_guard<std::mutex> lock(lfm::device_setup_mutex());
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&s->ev0));
     HIP_TRY(hipEventCreate(&s->ev1));

@@ -140,6 +140,9 @@ int create_fp32p(Solver *s) {
     s->d_cfl = s->d_scalar + 8;
     s->d_dtmin = s->d_scalar + 9;
     s->k64.partial = a.partial;  // shared reduction scratch for the final kernels
+    a.d_cfl = s->d_cfl;
+    a.d_dtmin = s->d_dtmin;
+    a.nblk_cells = nblocks((long)nc);
     HIP_TRY(hipMemcpy(s->d_scalar, &s->dt, sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -17,6 +19,11 @@ namespace lfm {
 struct KArgs32;
 
 int fail(int code, const std::string &msg);
+
+// Serialises device set-up (allocation, synchronous uploads) of new solvers
+// with hipGraph capture on other threads: a synchronous copy issued while a
+// capture is open would invalidate it.
+std::mutex &device_setup_mutex();
 
 struct Solver {
     lfm_params prm{};
@@ -79,3 +86,8 @@ int integrals(Solver *s, double *out_dev);
 int integrals64(Solver *s, double *out_dev);
 
 }  // namespace lfm
+
+// opaque handle of the C ABI
+struct lfm_solver {
+    std::unique_ptr<lfm::Solver> impl;
+};
