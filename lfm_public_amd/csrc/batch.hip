@@ -28,8 +28,8 @@ struct Batch {
     KArgs32 *d_args = nullptr;
     int32_t *d_maps = nullptr;
     size_t maps_cap = 0, args_cap = 0;
-    BatchMap map_bc{}, map_cells{}, map_faces{};
-    int nb_bc = 0, nb_cells = 0, nb_faces = 0;
+    BatchMap map_cells{}, map_faces{};
+    int nb_cells = 0, nb_faces = 0;
     int D = 2, solver = 0, minmod = 0, nstages = 0;
     double A[5]{}, B[5]{};
     hipGraph_t graph = nullptr;
@@ -48,7 +48,7 @@ template <int D>
 static void enqueue_step(Batch *b) {
     const dim3 blk(256);
     for (int rk = 0; rk < b->nstages; rk++) {
-        if (b->nb_bc) hipLaunchKernelGGL(bc_fill32_b<D>, dim3(b->nb_bc), blk, 0, b->stream, b->d_args, b->map_bc);
+        // ghosts are refreshed by the update kernel (members join with fresh ghosts)
         if (b->solver == 2 && b->minmod)
             hipLaunchKernelGGL((grad_vis32_b<D, 2>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args, b->map_cells);
         else
@@ -85,7 +85,7 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
     b->members.clear();
     if (n == 0) return LFM_OK;
     std::vector<KArgs32> args;
-    std::vector<int32_t> req_bc, req_c, req_f, first_bc, first_c, first_f;
+    std::vector<int32_t> req_c, req_f, first_c, first_f;
     for (int i = 0; i < n; i++) {
         if (!hs[i] || !hs[i]->impl) return fail(LFM_EINVAL, "null solver in batch");
         Solver *s = hs[i]->impl.get();
@@ -107,6 +107,10 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
         }
         for (Solver *o : b->members)
             if (o == s) return fail(LFM_EINVAL, "solver listed twice in a batch");
+        if (s->ghosts_stale) {
+            s->bc();
+            s->ghosts_stale = false;
+        }
         HIP_TRY(hipStreamSynchronize(s->stream));
         b->members.push_back(s);
         args.push_back(*s->k32);
@@ -115,14 +119,12 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
             const long nb = (count + 255) / 256;
             for (long k = 0; k < nb; k++) req.push_back(i);
         };
-        add(req_bc, first_bc, s->L.ng);
         add(req_c, first_c, s->L.nc);
         add(req_f, first_f, s->L.nf);
     }
-    b->nb_bc = (int)req_bc.size();
     b->nb_cells = (int)req_c.size();
     b->nb_faces = (int)req_f.size();
-    const size_t nmap = req_bc.size() + req_c.size() + req_f.size() + 3 * (size_t)n;
+    const size_t nmap = req_c.size() + req_f.size() + 2 * (size_t)n;
     if (nmap > b->maps_cap) {
         if (b->d_maps) HIP_TRY(hipFree(b->d_maps));
         HIP_TRY(hipMalloc((void **)&b->d_maps, nmap * sizeof(int32_t)));
@@ -140,8 +142,6 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
         maps.insert(maps.end(), v.begin(), v.end());
         return b->d_maps + off;
     };
-    b->map_bc = {put(req_bc), nullptr};
-    b->map_bc.first = put(first_bc);
     b->map_cells = {put(req_c), nullptr};
     b->map_cells.first = put(first_c);
     b->map_faces = {put(req_f), nullptr};

@@ -102,6 +102,7 @@ int Solver::get_field(int which, double *out) {
 int Solver::set_field(int which, const double *in) {
     const size_t nc = L.nc;
     HIP_TRY(hipStreamSynchronize(stream));
+    ghosts_stale = true;
     if (fp32) {
         if (which != LFM_FIELD_Q) return fail(LFM_EINVAL, "only q is settable on the FP32 path");
         KArgs32 &a = *k32;

@@ -48,6 +48,15 @@ struct KArgs32 {
 
 #define LERP32(w, a, b) ((w) * (a) + (1.0f - (w)) * (b))
 
+// FP32 reciprocal, quotient and square root on the hardware instructions
+// (v_rcp_f32, v_sqrt_f32: 1 ulp).  Spelled as target builtins so that every
+// kernel instantiating a body, single or batched, selects the same
+// instructions: the lowering of a plain `/` or sqrtf depends on whether its
+// operands are wave-uniform, which differs between the two launch forms.
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float div32(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
+__device__ __forceinline__ float sqrt32(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 template <int D>
 struct Prim32 {
     float rp, rho, ri, up[D], u[D], pp, Rp, R;
@@ -58,7 +67,7 @@ __device__ __forceinline__ Prim32<D> prim32(const KArgs32 &a, const float *q) {
     Prim32<D> s;
     s.rp = q[0];
     s.rho = a.rho_inf + s.rp;
-    s.ri = 1.0f / s.rho;
+    s.ri = rcp32(s.rho);
     float u2 = 0.f, du2 = 0.f;
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -103,7 +112,7 @@ __device__ __forceinline__ float minmod32(float cphi, float aphi, const float *g
     if (fabsf(gradcf) >= 1000.0f * fabsf(gradf))
         r = 2.0f * 1000.0f * sgn32(gradcf) * sgn32(gradf) - 1.0f;
     else
-        r = 2.0f * (gradcf / gradf) - 1.0f;
+        r = 2.0f * div32(gradcf, gradf) - 1.0f;
     const float m1 = (1.0f < r) ? 1.0f : r;
     const float lim = (m1 < 0.0f) ? 0.0f : m1;
     const float wt = lim * wl + (1.0f - lim) * (1.0f + flux) * 0.5f;
@@ -114,60 +123,65 @@ __device__ __forceinline__ float p5pos32(float M, float alpha) {
     const float M2Pos = 0.25f * (M + 1.0f) * (M + 1.0f);
     const float M2Neg = -0.25f * (M - 1.0f) * (M - 1.0f);
     if (fabsf(M) < 1.f) return M2Pos * ((2.0f - M) - 16.0f * alpha * M * M2Neg);
-    return 0.5f * (M + fabsf(M)) / M;
+    return div32(0.5f * (M + fabsf(M)), M);
 }
 
 __device__ __forceinline__ float p5neg32(float M, float alpha) {
     const float M2Pos = 0.25f * (M + 1.0f) * (M + 1.0f);
     const float M2Neg = -0.25f * (M - 1.0f) * (M - 1.0f);
     if (fabsf(M) < 1.f) return M2Neg * ((-2.0f - M) + 16.0f * alpha * M * M2Pos);
-    return 0.5f * (M - fabsf(M)) / M;
+    return div32(0.5f * (M - fabsf(M)), M);
 }
 
 // ---------------------------------------------------------------------------
 // ghost states (cfd_v0.cpp:1010-1133) in deviation form
 // ---------------------------------------------------------------------------
 template <int D>
-__device__ __forceinline__ void bc_fill32_body(const KArgs32 &a, const int g) {
-    if (g >= a.ng) return;
-    const int c = a.ghost_cell[g];
-    const int role = a.ghost_role[g];
-    float qi[D + 2], qg[D + 2];
-    load_q32<D>(a, c, qi);
+__device__ __forceinline__ bool ghost_state32(const KArgs32 &a, int role, const float *qi, float *qg) {
     if (role == 1) {  // wall: rho, -rhoU, rhoE
         qg[0] = qi[0];
 #pragma unroll
         for (int k = 0; k < D; k++) qg[k + 1] = -qi[k + 1] - 2.f * a.minf[k];
         qg[D + 1] = qi[D + 1];
-    } else if (role == 2 || role == 3) {
-        const Prim32<D> s = prim32<D>(a, qi);
-        if (role == 2) {  // inlet: T = T_inf, U = U_inf, p = p_int
-            const float rpg = s.rp + s.rho * (s.Rp * a.R_inv) / a.T_inf;
-            qg[0] = rpg;
-            float u2 = 0.f;
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                qg[k + 1] = rpg * a.uinf[k];
-                u2 += a.uinf[k] * a.uinf[k];
-            }
-            qg[D + 1] = s.pp / a.gm1 + 0.5f * rpg * u2;
-        } else {  // outlet: p = p_inf, U = U_int, rho = rho_int p_inf / p_int
-            const float pint = a.p_inf + s.pp;
-            const float rpg = (s.rp * a.p_inf - a.rho_inf * s.pp) / pint;
-            const float rhog = a.rho_inf + rpg;
-            float u2 = 0.f, du2 = 0.f;
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                qg[k + 1] = rpg * a.uinf[k] + rhog * s.up[k];
-                u2 += s.u[k] * s.u[k];
-                du2 += s.up[k] * (2.f * a.uinf[k] + s.up[k]);
-            }
-            qg[0] = rpg;
-            qg[D + 1] = 0.5f * (rpg * u2 + a.rho_inf * du2);
-        }
-    } else {
-        return;  // unset patch: ghost q = 0, i.e. q' = -q_inf (set at creation)
+        return true;
     }
+    if (role != 2 && role != 3) return false;  // unset patch: q = 0, i.e. q' = -q_inf
+    const Prim32<D> s = prim32<D>(a, qi);
+    if (role == 2) {  // inlet: T = T_inf, U = U_inf, p = p_int
+        const float rpg = s.rp + div32(s.rho * (s.Rp * a.R_inv), a.T_inf);
+        qg[0] = rpg;
+        float u2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            qg[k + 1] = rpg * a.uinf[k];
+            u2 += a.uinf[k] * a.uinf[k];
+        }
+        qg[D + 1] = div32(s.pp, a.gm1) + 0.5f * rpg * u2;
+    } else {  // outlet: p = p_inf, U = U_int, rho = rho_int p_inf / p_int
+        const float pint = a.p_inf + s.pp;
+        const float rpg = div32(s.rp * a.p_inf - a.rho_inf * s.pp, pint);
+        const float rhog = a.rho_inf + rpg;
+        float u2 = 0.f, du2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            qg[k + 1] = rpg * a.uinf[k] + rhog * s.up[k];
+            u2 += s.u[k] * s.u[k];
+            du2 += s.up[k] * (2.f * a.uinf[k] + s.up[k]);
+        }
+        qg[0] = rpg;
+        qg[D + 1] = 0.5f * (rpg * u2 + a.rho_inf * du2);
+    }
+    return true;
+}
+
+// ghost states (cfd_v0.cpp:1010-1133) in deviation form; the fused update
+// refreshes ghosts, so this runs only after the state is set from outside
+template <int D>
+__device__ __forceinline__ void bc_fill32_body(const KArgs32 &a, const int g) {
+    if (g >= a.ng) return;
+    float qi[D + 2], qg[D + 2];
+    load_q32<D>(a, a.ghost_cell[g], qi);
+    if (!ghost_state32<D>(a, a.ghost_role[g], qi, qg)) return;
     const int gi = a.nc + g;
 #pragma unroll
     for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + gi] = qg[k];
@@ -187,19 +201,24 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
 #pragma unroll
         for (int j = 0; j < D; j++) dudx[i][j] = ug[i][j] = 0.f;
     }
+    float qc0[D + 2];
+    load_q32<D>(a, c, qc0);
+    const Prim32<D> own = prim32<D>(a, qc0);
     const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
     for (int e = e0; e < e1; e++) {
         const int ent = a.g_ent[e];
         const bool isP = ent >= 0;
-        const int f = isP ? ent : ~ent;
-        const int P = a.fP[f], N = a.fN[f];
+        const int f = isP ? (ent & kFaceMask) : ~ent;
+        // the face's other cell; the face values use the face's own P/N order
+        const int other = isP ? a.fN[f] : a.fP[f];
         const float w = a.w[f];
-        float S[D], qP[D + 2], qN[D + 2];
+        float S[D], qo[D + 2];
 #pragma unroll
         for (int k = 0; k < D; k++) S[k] = a.S[(size_t)k * a.nf + f];
-        load_q32<D>(a, P, qP);
-        load_q32<D>(a, N, qN);
-        const Prim32<D> sp = prim32<D>(a, qP), sn = prim32<D>(a, qN);
+        load_q32<D>(a, other, qo);
+        const Prim32<D> so = prim32<D>(a, qo);
+        const Prim32<D> &sp = isP ? own : so;
+        const Prim32<D> &sn = isP ? so : own;
         float sv[D];
 #pragma unroll
         for (int k = 0; k < D; k++) sv[k] = isP ? S[k] * vinv : -S[k] * vinv;
@@ -225,9 +244,7 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
             }
         }
     }
-    float qc[D + 2];
-    load_q32<D>(a, c, qc);
-    const Prim32<D> sc = prim32<D>(a, qc);
+    const Prim32<D> &sc = own;
     const float mu = a.mu;
     float tau[D][D];
 #pragma unroll
@@ -291,7 +308,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
     float Sm2 = 0.f;
 #pragma unroll
     for (int k = 0; k < D; k++) Sm2 += S[k] * S[k];
-    const float Sm = sqrtf(Sm2);
+    const float Sm = sqrt32(Sm2);
     float phi_inf = 0.f;
 #pragma unroll
     for (int k = 0; k < D; k++) phi_inf += a.uinf[k] * S[k];
@@ -302,7 +319,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
         // (hence p and e = Rpsi/(g-1)) interpolated linearly
         const float rp = LERP32(w, qP[0], qN[0]);
         const float rho = a.rho_inf + rp;
-        const float ri = 1.f / rho;
+        const float ri = rcp32(rho);
         float mp[D], up[D];
         float phip = 0.f, du2 = 0.f;
 #pragma unroll
@@ -331,7 +348,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
         // midpoint averages (cfd_v0.cpp:1968-2023)
         const float rp = 0.5f * (qP[0] + qN[0]);
         const float rho = a.rho_inf + rp;
-        const float ri = 1.f / rho;
+        const float ri = rcp32(rho);
         float mp[D], up[D];
         float phip = 0.f;
 #pragma unroll
@@ -353,7 +370,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
         const float Hp = 0.5f * ((a.gog * sp.Rp + 0.5f * duP) + (a.gog * sn.Rp + 0.5f * duN));
         const float Gdev = rp * a.H_inf + rho * Hp;
         if (SOLVER == 2) {
-            const float cP = sqrtf(a.gamma * sp.R), cN = sqrtf(a.gamma * sn.R);
+            const float cP = sqrt32(a.gamma * sp.R), cN = sqrt32(a.gamma * sn.R);
             const float cavg = 0.5f * (cP + cN);
             float cdiv = 0.f, ndiv = 0.f, ccurl = 0.f, ncurl = 0.f, unP = 0.f, unN = 0.f;
             float cdu[D][D], ndu[D][D];
@@ -382,8 +399,8 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
                 unP += minmod32<D>(sp.up[p], sn.up[p], gc, d, w, 1.0f) * S[p];
                 unN += minmod32<D>(sp.up[p], sn.up[p], gn, d, w, -1.0f) * S[p];
             }
-            const float cth0 = -(cdiv / sqrtf(cdiv * cdiv + ccurl + 4e-2f));
-            const float nth0 = -(ndiv / sqrtf(ndiv * ndiv + ncurl + 4e-2f));
+            const float cth0 = -div32(cdiv, sqrt32(cdiv * cdiv + ccurl + 4e-2f));
+            const float nth0 = -div32(ndiv, sqrt32(ndiv * ndiv + ncurl + 4e-2f));
             const float tha = 0.5f * (((cth0 < 0.f) ? 0.f : cth0) + ((nth0 < 0.f) ? 0.f : nth0));
             float gc[D], gn[D], hc[D], hn[D];
 #pragma unroll
@@ -398,12 +415,12 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
             const float pPd = minmod32<D>(sp.pp, sn.pp, hc, d, w, 1.0f);
             const float pNd = minmod32<D>(sp.pp, sn.pp, hn, d, w, -1.0f);
             const float unPf = phi_inf + unP, unNf = phi_inf + unN;
-            const float Msq = (unPf * unPf + unNf * unNf) / (2.f * cavg * cavg * Sm2);
-            const float MP = unPf / (Sm * cavg);
-            const float MN = unNf / (Sm * cavg);
+            const float Msq = div32(unPf * unPf + unNf * unNf, 2.f * cavg * cavg * Sm2);
+            const float MP = div32(unPf, Sm * cavg);
+            const float MN = div32(unNf, Sm * cavg);
             const float Minf = 0.2f;
             const float mx = (Msq < Minf * Minf) ? Minf * Minf : Msq;
-            const float M0 = sqrtf((mx < 1.0f) ? mx : 1.0f);
+            const float M0 = sqrt32((mx < 1.0f) ? mx : 1.0f);
             const float fa = M0 * (2.0f - M0);
             const float alpha = 3.0f * (5.0f * fa * fa - 4.0f) / 16.0f;
             const float dN = p5pos32(MN, alpha) - p5neg32(MN, alpha);
@@ -418,7 +435,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
                 const float s4 = MN * MN * MN * MN + MN * MN * MN * MP + MN * MN * MP * MP + MN * MP * MP * MP +
                                  MP * MP * MP * MP;
                 const float dd = 0.5f * (3.f - s2) + 2.f * alpha * (1.f - 2.f * s2 + s4);
-                dNP = ((unN - unP) / (Sm * cavg)) * dd;
+                dNP = div32(unN - unP, Sm * cavg) * dd;
             } else {
                 dNP = dN - dP;
             }
@@ -441,7 +458,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
     float dmag2 = 0.f;
 #pragma unroll
     for (int k = 0; k < D; k++) dmag2 += d[k] * d[k];
-    const float dmi = 1.f / sqrtf(dmag2);
+    const float dmi = rcp32(sqrt32(dmag2));
     float dn[D], du[D][D], dT[D], tau[D][D], tmc[D][D];
 #pragma unroll
     for (int k = 0; k < D; k++) dn[k] = d[k] * dmi;
@@ -495,11 +512,11 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
     float K[D], dm2 = 0.f;
 #pragma unroll
     for (int i = 0; i < D; i++) {
-        const float delta = d[i] * Sm2 / Sd;
+        const float delta = div32(d[i] * Sm2, Sd);
         dm2 += delta * delta;
         K[i] = S[i] - delta;
     }
-    const float dm = sqrtf(dm2);
+    const float dm = sqrt32(dm2);
     float sU[D];
     if (ghost) {
         float Uf[D];
@@ -541,9 +558,11 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, float A, fl
             res[k] = 0.f;
         }
         const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
+        int gface[kMaxFaces], ngf = 0;
         for (int e = e0; e < e1; e++) {
             const int ent = a.g_ent[e];
-            const int f = ent >= 0 ? ent : ~ent;
+            const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
+            if (ent >= 0 && (ent & kGhostFaceFlag) && ngf < kMaxFaces) gface[ngf++] = f;
             const float sg = ent >= 0 ? 1.f : -1.f;
 #pragma unroll
             for (int k = 0; k < D + 2; k++) {
@@ -561,6 +580,14 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, float A, fl
             q[k] += B * dq[k];
             a.q[(size_t)k * a.ntot + c] = q[k];
             a.dq[(size_t)k * a.nc + c] = dq[k];
+        }
+        // refresh this cell's boundary ghosts for the next stage
+        for (int i = 0; i < ngf; i++) {
+            const int g = a.fN[gface[i]] - a.nc;
+            float qg[D + 2];
+            if (!ghost_state32<D>(a, a.ghost_role[g], q, qg)) continue;
+#pragma unroll
+            for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + a.nc + g] = qg[k];
         }
         if (RES) {
 #pragma unroll
@@ -698,23 +725,23 @@ __global__ void __launch_bounds__(256) count_nonfinite32(const float *q, int nc,
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
-    bc_fill32_body<D>(a, blockIdx.x * blockDim.x + threadIdx.x);
+    bc_fill32_body<D>(a, LFM_TID(blockIdx.x));
 }
 template <int D, int GRAD>
 __global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
-    grad_vis32_body<D, GRAD>(a, blockIdx.x * blockDim.x + threadIdx.x);
+    grad_vis32_body<D, GRAD>(a, LFM_TID(blockIdx.x));
 }
 template <int D, int SOLVER>
 __global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
-    face_flux32_body<D, SOLVER>(a, blockIdx.x * blockDim.x + threadIdx.x);
+    face_flux32_body<D, SOLVER>(a, LFM_TID(blockIdx.x));
 }
 template <int D, bool RES>
 __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B, int zero_dq) {
-    cell_update32_body<D, RES>(a, A, B, zero_dq, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.x);
+    cell_update32_body<D, RES>(a, A, B, zero_dq, LFM_TID(blockIdx.x), LFM_BLK());
 }
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
-    cfl_dt_cells32_body<D, MODE>(a, param, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.x);
+    cfl_dt_cells32_body<D, MODE>(a, param, LFM_TID(blockIdx.x), LFM_BLK());
 }
 
 // ---------------------------------------------------------------------------
@@ -729,16 +756,11 @@ struct BatchMap {
 };
 
 __device__ __forceinline__ void batch_slot(const BatchMap &m, int &r, int &lblk) {
-    r = m.req[blockIdx.x];
-    lblk = (int)blockIdx.x - m.first[r];
+    const int b = LFM_BLK();
+    r = m.req[b];
+    lblk = b - m.first[r];
 }
 
-template <int D>
-__global__ void __launch_bounds__(256) bc_fill32_b(const KArgs32 *args, BatchMap m) {
-    int r, lb;
-    batch_slot(m, r, lb);
-    bc_fill32_body<D>(args[r], lb * 256 + threadIdx.x);
-}
 template <int D, int GRAD>
 __global__ void __launch_bounds__(256) grad_vis32_b(const KArgs32 *args, BatchMap m) {
     int r, lb;

@@ -16,6 +16,8 @@
 
 #include <cstdint>
 
+#include "mesh_builder.hpp"
+
 namespace lfm {
 
 struct KArgs {
@@ -40,6 +42,19 @@ struct KArgs {
 };
 
 #define INTERP_LINEAR(w, a, b) ((w) * (a) + (1.0 - (w)) * (b))
+
+// MI355X dispatches workgroup b to XCD b % 8, and each XCD has its own L2.
+// Remap so that every XCD runs one contiguous slice of the index space: the
+// neighbours a face or cell gathers are then mostly in that XCD's L2.
+// Bijection of [0, nb).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b % kXcds, j = b / kXcds;
+    const int per = nb / kXcds, rem = nb % kXcds;
+    return x * per + (x < rem ? x : rem) + j;
+}
+#define LFM_TID(blk) (xcd_block((int)(blk), (int)gridDim.x) * (int)blockDim.x + (int)threadIdx.x)
+#define LFM_BLK() xcd_block((int)blockIdx.x, (int)gridDim.x)
 
 template <int D>
 __device__ __forceinline__ double rpsi(const KArgs &a, const double *q) {
@@ -102,61 +117,71 @@ __device__ __forceinline__ double p5neg(double M, double alpha) {
 }
 
 // ---------------------------------------------------------------------------
-// set_boundary_conditions (cfd_v0.cpp:1010-1133): ghost state per boundary
-// face from its interior cell.  Ghost gradient copies are not stored: the
-// interior gradients they copy are zero at this point of every stage, and
-// the flux kernels read zero ghost gradients (see grad arrays' ghost part).
+// Ghost state of one boundary face from its interior cell's state qi
+// (cfd_v0.cpp:1010-1133); returns false for an unset patch (the ghost keeps
+// q = 0).
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void __launch_bounds__(256) bc_fill(KArgs a) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.ng) return;
-    const int c = a.ghost_cell[g];
-    const int role = a.ghost_role[g];
-    double qi[D + 2], qg[D + 2];
-    load_q<D>(a, c, qi);
+__device__ __forceinline__ bool ghost_state(const KArgs &a, int role, const double *qi, double *qg) {
     if (role == 1) {  // wall
         qg[0] = qi[0];
 #pragma unroll
         for (int k = 0; k < D; k++) qg[k + 1] = -qi[k + 1];
         qg[D + 1] = qi[D + 1];
-    } else if (role == 2 || role == 3) {  // inlet / outlet
-        const double rhoInt = qi[0];
-        double Uint[D], u2i = 0.0;
+        return true;
+    }
+    if (role != 2 && role != 3) return false;
+    const double rhoInt = qi[0];
+    double Uint[D], u2i = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+        Uint[k] = qi[k + 1] / rhoInt;
+        u2i += Uint[k] * Uint[k];
+    }
+    const double pInt = (qi[D + 1] - 0.5 * u2i * rhoInt) * a.gm1;
+    double rhoExt, pExt, Uext[D], u2e = 0.0;
+    if (role == 2) {  // inlet
+        const double TInt = pInt * a.R_inv / rhoInt;
+        const double TExt = a.T_inf;
+        rhoExt = rhoInt * TInt / TExt;
 #pragma unroll
         for (int k = 0; k < D; k++) {
-            Uint[k] = qi[k + 1] / rhoInt;
-            u2i += Uint[k] * Uint[k];
+            Uext[k] = a.U_inf[k];
+            u2e += Uext[k] * Uext[k];
         }
-        const double pInt = (qi[D + 1] - 0.5 * u2i * rhoInt) * a.gm1;
-        double rhoExt, pExt, Uext[D], u2e = 0.0;
-        if (role == 2) {
-            const double TInt = pInt * a.R_inv / rhoInt;
-            const double TExt = a.T_inf;
-            rhoExt = rhoInt * TInt / TExt;
+        pExt = pInt;
+    } else {  // outlet
+        pExt = a.p_inf;
+        rhoExt = rhoInt * pExt / pInt;
 #pragma unroll
-            for (int k = 0; k < D; k++) {
-                Uext[k] = a.U_inf[k];
-                u2e += Uext[k] * Uext[k];
-            }
-            pExt = pInt;
-        } else {
-            pExt = a.p_inf;
-            rhoExt = rhoInt * pExt / pInt;
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                Uext[k] = Uint[k];
-                u2e += Uext[k] * Uext[k];
-            }
+        for (int k = 0; k < D; k++) {
+            Uext[k] = Uint[k];
+            u2e += Uext[k] * Uext[k];
         }
-        const double EExt = pExt / (rhoExt * a.gm1) + 0.5 * u2e;
-        qg[0] = rhoExt;
-#pragma unroll
-        for (int k = 0; k < D; k++) qg[k + 1] = rhoExt * Uext[k];
-        qg[D + 1] = rhoExt * EExt;
-    } else {
-        return;  // unset patch: ghost keeps q = 0 (as the reference's zeroed ghosts)
     }
+    const double EExt = pExt / (rhoExt * a.gm1) + 0.5 * u2e;
+    qg[0] = rhoExt;
+#pragma unroll
+    for (int k = 0; k < D; k++) qg[k + 1] = rhoExt * Uext[k];
+    qg[D + 1] = rhoExt * EExt;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// set_boundary_conditions (cfd_v0.cpp:1010-1133): ghost state per boundary
+// face from its interior cell.  Ghost gradient copies are not stored: the
+// interior gradients they copy are zero at this point of every stage, and
+// the flux kernels read zero ghost gradients (see grad arrays' ghost part).
+// In the fused step the update kernel refreshes the ghosts of the cells it
+// updates, so this kernel only runs after the state is set from outside.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256) bc_fill(KArgs a) {
+    const int g = LFM_TID(blockIdx.x);
+    if (g >= a.ng) return;
+    double qi[D + 2], qg[D + 2];
+    load_q<D>(a, a.ghost_cell[g], qi);
+    if (!ghost_state<D>(a, a.ghost_role[g], qi, qg)) return;
     const int gi = a.nc + g;
 #pragma unroll
     for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + gi] = qg[k];
@@ -172,7 +197,7 @@ __global__ void __launch_bounds__(256) bc_fill(KArgs a) {
 // ---------------------------------------------------------------------------
 template <int D, int GRAD>
 __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = LFM_TID(blockIdx.x);
     if (c >= a.nc) return;
     const double vinv = a.vinv[c];
     double dudx[D][D], dTdx[D];
@@ -188,20 +213,28 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
             ug[i][j] = 0.0;
         }
     }
+    double qself[D + 2];
+    load_q<D>(a, c, qself);
+    const double Rself = rpsi<D>(a, qself), riself = 1.0 / qself[0];
     const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
     for (int e = e0; e < e1; e++) {
         const int ent = a.g_ent[e];
         const bool isP = ent >= 0;
-        const int f = isP ? ent : ~ent;
-        const int P = a.fP[f], N = a.fN[f];
+        const int f = isP ? (ent & kFaceMask) : ~ent;
+        const int other = isP ? a.fN[f] : a.fP[f];
         const double w = a.w[f];
-        double S[D], qP[D + 2], qN[D + 2];
+        double S[D], qo[D + 2], qP[D + 2], qN[D + 2];
 #pragma unroll
         for (int k = 0; k < D; k++) S[k] = a.S[(size_t)k * a.nf + f];
-        load_q<D>(a, P, qP);
-        load_q<D>(a, N, qN);
-        const double cR = rpsi<D>(a, qP), aR = rpsi<D>(a, qN);
-        const double cri = 1.0 / qP[0], ari = 1.0 / qN[0];
+        load_q<D>(a, other, qo);
+        const double Ro = rpsi<D>(a, qo), rio = 1.0 / qo[0];
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            qP[k] = isP ? qself[k] : qo[k];
+            qN[k] = isP ? qo[k] : qself[k];
+        }
+        const double cR = isP ? Rself : Ro, aR = isP ? Ro : Rself;
+        const double cri = isP ? riself : rio, ari = isP ? rio : riself;
         double sv[D];
 #pragma unroll
         for (int k = 0; k < D; k++) sv[k] = isP ? S[k] * vinv : -S[k] * vinv;
@@ -241,8 +274,7 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
         }
     }
     // refresh of tau / sigmaU / tauMC from the completed dudx (cfd_v0.cpp:1813-1834)
-    double qc[D + 2];
-    load_q<D>(a, c, qc);
+    const double *qc = qself;
     double U[D];
 #pragma unroll
     for (int k = 0; k < D; k++) U[k] = qc[k + 1] / qc[0];
@@ -293,7 +325,7 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
 // ---------------------------------------------------------------------------
 template <int D, int SOLVER>
 __global__ void __launch_bounds__(256) face_flux(KArgs a) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = LFM_TID(blockIdx.x);
     if (f >= a.nf) return;
     const int P = a.fP[f], N = a.fN[f];
     const bool ghost = N >= a.nc;
@@ -565,7 +597,7 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
 // ---------------------------------------------------------------------------
 template <int D, bool RES>
 __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, int zero_dq) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = LFM_TID(blockIdx.x);
     __shared__ double red[256 / 64][D + 2];
     double rsq[D + 2];
 #pragma unroll
@@ -580,12 +612,15 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
             res[k] = 0.0;
         }
         const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
+        int gface[kMaxFaces], ngf = 0;
         for (int e = e0; e < e1; e++) {
             const int ent = a.g_ent[e];
             if (ent >= 0) {
+                const int f = ent & kFaceMask;
+                if ((ent & kGhostFaceFlag) && ngf < kMaxFaces) gface[ngf++] = f;
 #pragma unroll
                 for (int k = 0; k < D + 2; k++) {
-                    const double r = a.frhs[(size_t)k * a.nf + ent];
+                    const double r = a.frhs[(size_t)k * a.nf + f];
                     if (RES) res[k] += r;
                     dq[k] += dt * r * vinv;
                 }
@@ -612,6 +647,14 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
             a.q[(size_t)k * a.ntot + c] = q[k];
             a.dq[(size_t)k * a.nc + c] = dq[k];
         }
+        // refresh this cell's boundary ghosts for the next stage
+        for (int i = 0; i < ngf; i++) {
+            const int g = a.fN[gface[i]] - a.nc;
+            double qg[D + 2];
+            if (!ghost_state<D>(a, a.ghost_role[g], q, qg)) continue;
+#pragma unroll
+            for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + a.nc + g] = qg[k];
+        }
         if (RES) {
 #pragma unroll
             for (int k = 0; k < D + 2; k++) {
@@ -635,7 +678,7 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
             for (int k = 0; k < D + 2; k++) {
                 double s = 0.0;
                 for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i][k];
-                a.partial[(size_t)blockIdx.x * (D + 2) + k] = s;
+                a.partial[(size_t)LFM_BLK() * (D + 2) + k] = s;
             }
         }
     }
@@ -664,7 +707,7 @@ static __global__ void __launch_bounds__(256) reduce_sum(const double *partial, 
 // ---------------------------------------------------------------------------
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = LFM_TID(blockIdx.x);
     __shared__ double sh[256];
     double v = MODE == 0 ? 100.0 : 0.0;
     if (c < a.nc) {
@@ -673,7 +716,7 @@ __global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
         double sum = 0.0;
         for (int e = a.s_off[c]; e < a.s_off[c + 1]; e++) {
             const int ent = a.s_ent[e];
-            const int f = ent >= 0 ? ent : ~ent;
+            const int f = ent >= 0 ? ent : ~ent;  // slot lists carry no ghost flag
             double S[D];
 #pragma unroll
             for (int k = 0; k < D; k++) S[k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
@@ -715,7 +758,7 @@ __global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) a.partial[LFM_BLK()] = sh[0];
 }
 
 template <int MODE>

@@ -378,6 +378,7 @@ std::string build_layout(const lfm_polymesh &m, const lfm_params &prm, const dou
                 L.nf++;
             }
     const int NF = L.nf;
+    if (NF >= kGhostFaceFlag) return "too many faces for one partition (2^30)";
     L.S.assign((size_t)D * NF, 0.0);
     L.d.assign((size_t)D * NF, 0.0);
     L.w.assign(NF, 0.0);
@@ -433,7 +434,7 @@ std::string build_layout(const lfm_polymesh &m, const lfm_params &prm, const dou
         std::fill(cnt.begin(), cnt.end(), 0);
         for (int f = 0; f < NF; f++) {
             int P = L.fP[f], N = L.fN[f];
-            L.g_ent[L.g_off[P] + cnt[P]++] = f;
+            L.g_ent[L.g_off[P] + cnt[P]++] = N < nc ? f : (f | kGhostFaceFlag);
             if (N < nc) L.g_ent[L.g_off[N] + cnt[N]++] = ~f;
         }
     }

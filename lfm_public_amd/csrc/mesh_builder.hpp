@@ -12,6 +12,10 @@ namespace lfm {
 
 constexpr int kMaxFaces = 6;        // FACE_CNT in {3,4,5,6} (src/mesh_solver.cpp:56-74)
 constexpr int kGhostSubmesh = 10000; // MAX_MPI_RANKS, api/mpi_env.h:4
+// gather-list encoding: entry f >= 0 (cell is P), ~f (cell is N); P entries
+// of faces whose N is a boundary ghost carry kGhostFaceFlag.
+constexpr int32_t kGhostFaceFlag = 1 << 30;
+constexpr int32_t kFaceMask = kGhostFaceFlag - 1;
 
 // The reference's two submeshes of one partition, flattened into one index
 // space: cells [0, nsm0) are submesh 0 (boundary), [nsm0, nc) submesh 1,

@@ -280,9 +280,14 @@ void Solver::prof_mark(int grp, bool start) {
 void Solver::step() {
     const bool res = prm.have_res != 0;
     for (int rk = 0; rk < nstages; rk++) {
-        prof_mark(0, true);
-        bc();
-        prof_mark(0, false);
+        // the update kernel refreshes the ghosts; a full fill is needed only
+        // after the state was set from outside
+        if (ghosts_stale) {
+            prof_mark(0, true);
+            bc();
+            prof_mark(0, false);
+            ghosts_stale = false;
+        }
         prof_mark(1, true);
         gradvis();
         prof_mark(1, false);
@@ -405,6 +410,7 @@ int lfm_prepare_for_rkstep(lfm_solver *h, int rk) {
 int lfm_set_boundary_conditions(lfm_solver *h) {
     if (!h) return fail(LFM_EINVAL, "null handle");
     h->impl->bc();
+    h->impl->ghosts_stale = false;
     HIP_TRY(hipGetLastError());
     return LFM_OK;
 }

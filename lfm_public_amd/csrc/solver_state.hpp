@@ -56,6 +56,7 @@ struct Solver {
     size_t prof_n = 0;
     void prof_mark(int grp, bool start);
 
+    bool ghosts_stale = true;  // ghost states not derived from the current q
     double pending_A = 1.0;
     int pending_rk = 0;
     int64_t steps_done = 0;
