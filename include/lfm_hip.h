@@ -38,6 +38,23 @@ extern "C" {
 #define LFM_BC_OUTLET 3
 #define LFM_BC_EMPTY 4
 #define LFM_BC_CYCLIC 5
+#define LFM_BC_PROCESSOR 6 /* processor patch of a decomposed case: patch_nbr
+                              holds the neighbour partition's rank */
+
+/* Halo payloads (MPI_EXCHANGE_SPLIT / _PACKED, api/mpi_env.h:34-38): per
+ * cell record, in the reference's send-buffer order (cfd_v0.cpp:3303-3331). */
+#define LFM_HALO_SOLVARS 1 /* q: D+2                                        */
+#define LFM_HALO_VISCOUS 2 /* dudx (D*D), dTdx (D), tauMC (D*D), sigmaU (D) */
+#define LFM_HALO_PACKED 3  /* SOLVARS then VISCOUS                          */
+
+/* One partition of a decomposed case (decomposePar): its rank and, per
+ * boundary face (index f - n_internal), the global face key that both sides
+ * of a processor patch share (|faceProcAddressing|,
+ * polyMeshReaderOF.cpp:251-260).  Only processor-patch faces are read. */
+typedef struct {
+    int32_t rank;
+    const int32_t *face_key;
+} lfm_partition;
 
 /* Precision of the device state. */
 #define LFM_PREC_FP64 64  /* plain FP64, the reference's double build      */
@@ -115,6 +132,14 @@ int lfm_patch_role(const char *type, const char *name);
  * alpha may be NULL when Ls <= 0. */
 int lfm_solver_create(const lfm_polymesh *mesh, const lfm_params *prm, const double *p, const double *T,
                       const double *U3, const double *alpha, lfm_solver **out);
+/* The same for one partition of a decomposed case: processor patches
+ * (LFM_BC_PROCESSOR) get one halo ghost per face, filled by
+ * lfm_halo_unpack.  The handle cannot step until lfm_halo_import has run
+ * for every neighbour (the processor faces' d and w need the neighbour's
+ * cell centres). */
+int lfm_solver_create_part(const lfm_polymesh *mesh, const lfm_partition *part, const lfm_params *prm,
+                           const double *p, const double *T, const double *U3, const double *alpha,
+                           lfm_solver **out);
 int lfm_solver_destroy(lfm_solver *s);
 const char *lfm_last_error(void);
 int lfm_get_info(lfm_solver *s, lfm_info *info);
@@ -160,6 +185,45 @@ int lfm_get_residual(lfm_solver *s, double *res);
 int lfm_conservation(lfm_solver *s, double *out);
 /* Number of non-finite conservatives in the state (0 = healthy). */
 int lfm_check_finite(lfm_solver *s, int64_t *n_bad);
+
+/* ---------------------------------------------------------------------
+ * Halo exchange between partitions (ISolver::mpi_communication / mpi_wait,
+ * cfd_v0.cpp:3284-3692; set-up messages of calculateVNeighbour,
+ * mesh_reader.cpp:512-612, and assign_pointers, cfd_v0.cpp:616-700).
+ * --------------------------------------------------------------------- */
+/* Neighbour ranks, in order of first appearance over the boundary-submesh
+ * cells (ranks may be NULL to query n only). */
+int lfm_halo_neighbours(lfm_solver *s, int32_t *n, int32_t *ranks);
+/* The set-up message for neighbour `rank`: per processor face toward it,
+ * the face key, the local owner cell (boundary-submesh index) and the
+ * displacement cell centre - face centre (3 doubles).  NULL arrays: size
+ * only. */
+int lfm_halo_export(lfm_solver *s, int32_t rank, int32_t *n_faces, int32_t *keys, int32_t *cells, double *disp);
+/* Apply the message lfm_halo_export produced on partition `rank`. */
+int lfm_halo_import(lfm_solver *s, int32_t rank, int32_t n_faces, const int32_t *keys, const int32_t *cells,
+                    const double *disp);
+/* Bytes of one payload to send to / receive from neighbour `rank`
+ * (8-byte values for FP64 solvers, 4-byte deviations for FP32P). */
+int lfm_halo_sizes(lfm_solver *s, int32_t rank, int kind, int64_t *send_bytes, int64_t *recv_bytes);
+/* Pack the send list's records into the device buffer dev_buf, and unpack
+ * a received buffer into the halo ghosts; both are ordered on the handle's
+ * stream (synchronise the sender before the receiver reads). */
+int lfm_halo_pack(lfm_solver *s, int32_t rank, int kind, void *dev_buf);
+int lfm_halo_unpack(lfm_solver *s, int32_t rank, int kind, const void *dev_buf);
+
+/* Several partitions of one case resident on this device, stepped as
+ * Mesh::solve steps one rank each (no counterpart as an object: the
+ * reference runs one MPI rank per partition): per stage, gradients and
+ * viscous tensors of every partition, viscous halo exchange, fluxes and
+ * update, solvars exchange; per step the dt minimum over partitions when
+ * adjustTimeStep is set.  parts[i] must be the partition of rank i with
+ * every neighbour imported; all of one precision and scheme.  While in a
+ * group, a partition must not be stepped on its own. */
+typedef struct lfm_part_group lfm_part_group;
+int lfm_part_group_create(lfm_solver *const *parts, int n, lfm_part_group **out);
+int lfm_part_group_run(lfm_part_group *g, int nsteps);
+int lfm_part_group_synchronize(lfm_part_group *g);
+int lfm_part_group_destroy(lfm_part_group *g);
 
 /* ---------------------------------------------------------------------
  * Shared launches over several resident solvers (no counterpart in the

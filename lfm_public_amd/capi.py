@@ -28,7 +28,11 @@ EXPORTS = ("lfm_patch_role", "lfm_solver_create", "lfm_solver_destroy", "lfm_las
            "lfm_calc_gradients", "lfm_calc_vis", "lfm_one_rk_step", "lfm_compute_dt", "lfm_compute_cfl", "lfm_run",
            "lfm_synchronize", "lfm_time_steps", "lfm_get_field", "lfm_set_field", "lfm_get_residual",
            "lfm_check_finite", "lfm_profile_steps", "lfm_conservation", "lfm_batch_create", "lfm_batch_destroy",
-           "lfm_batch_set", "lfm_batch_run", "lfm_batch_synchronize")
+           "lfm_batch_set", "lfm_batch_run", "lfm_batch_synchronize", "lfm_solver_create_part",
+           "lfm_halo_neighbours", "lfm_halo_export", "lfm_halo_import", "lfm_halo_sizes", "lfm_halo_pack",
+           "lfm_halo_unpack", "lfm_part_group_create", "lfm_part_group_run", "lfm_part_group_synchronize",
+           "lfm_part_group_destroy")
+HALO_SOLVARS, HALO_VISCOUS, HALO_PACKED = 1, 2, 3
 
 
 class LfmError(RuntimeError):
@@ -43,6 +47,10 @@ class PolyMeshDesc(C.Structure):
                 ("n_internal", C.c_int32), ("neighbour", C.c_void_p), ("n_cells", C.c_int32),
                 ("n_patches", C.c_int32), ("patch_start", C.c_void_p), ("patch_size", C.c_void_p),
                 ("patch_role", C.c_void_p), ("patch_nbr", C.c_void_p), ("cell_submesh", C.c_void_p)]
+
+
+class PartitionDesc(C.Structure):
+    _fields_ = [("rank", C.c_int32), ("face_key", C.c_void_p)]
 
 
 class Params(C.Structure):
@@ -76,6 +84,18 @@ def load() -> C.CDLL:
     L.lfm_last_error.restype = C.c_char_p
     L.lfm_patch_role.argtypes = [C.c_char_p, C.c_char_p]
     L.lfm_solver_create.argtypes = [C.POINTER(PolyMeshDesc), C.POINTER(Params), vp, vp, vp, vp, C.POINTER(vp)]
+    L.lfm_solver_create_part.argtypes = [C.POINTER(PolyMeshDesc), C.POINTER(PartitionDesc), C.POINTER(Params), vp,
+                                         vp, vp, vp, C.POINTER(vp)]
+    L.lfm_halo_neighbours.argtypes = [vp, C.POINTER(i32), vp]
+    L.lfm_halo_export.argtypes = [vp, i32, C.POINTER(i32), vp, vp, vp]
+    L.lfm_halo_import.argtypes = [vp, i32, i32, vp, vp, vp]
+    L.lfm_halo_sizes.argtypes = [vp, i32, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.lfm_halo_pack.argtypes = [vp, i32, C.c_int, vp]
+    L.lfm_halo_unpack.argtypes = [vp, i32, C.c_int, vp]
+    L.lfm_part_group_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.lfm_part_group_run.argtypes = [vp, C.c_int]
+    L.lfm_part_group_synchronize.argtypes = [vp]
+    L.lfm_part_group_destroy.argtypes = [vp]
     L.lfm_solver_destroy.argtypes = [vp]
     L.lfm_get_info.argtypes = [vp, C.POINTER(Info)]
     for n in ("lfm_prepare_for_timestep", "lfm_set_boundary_conditions", "lfm_calc_gradients", "lfm_calc_vis",
@@ -168,7 +188,15 @@ class GpuSolver:
         aa = arr(alpha, np.float64) if alpha is not None else None
         h = C.c_void_p()
         self.h = None
-        _check(L.lfm_solver_create(C.byref(m), C.byref(pr), _ptr(pa), _ptr(Ta), _ptr(Ua), _ptr(aa), C.byref(h)))
+        if mesh.face_key is not None:
+            part = PartitionDesc()
+            part.rank = int(mesh.rank)
+            part.face_key = _ptr(arr(mesh.face_key, np.int32))
+            _check(L.lfm_solver_create_part(C.byref(m), C.byref(part), C.byref(pr), _ptr(pa), _ptr(Ta), _ptr(Ua),
+                                            _ptr(aa), C.byref(h)))
+        else:
+            _check(L.lfm_solver_create(C.byref(m), C.byref(pr), _ptr(pa), _ptr(Ta), _ptr(Ua), _ptr(aa),
+                                       C.byref(h)))
         self.h = h.value
 
     @classmethod
@@ -273,6 +301,77 @@ class GpuSolver:
         n = C.c_int64()
         _check(load().lfm_check_finite(self.h, C.byref(n)))
         return n.value
+
+    # halo (processor patches) ----------------------------------------------
+    def halo_neighbours(self):
+        n = C.c_int32()
+        _check(load().lfm_halo_neighbours(self.h, C.byref(n), None))
+        out = np.zeros(max(1, n.value), np.int32)
+        _check(load().lfm_halo_neighbours(self.h, C.byref(n), _ptr(out)))
+        return out[: n.value].tolist()
+
+    def halo_export(self, rank: int):
+        """Set-up message for neighbour ``rank``: (keys, cells, disp[n, 3])."""
+        n = C.c_int32()
+        _check(load().lfm_halo_export(self.h, rank, C.byref(n), None, None, None))
+        keys = np.zeros(n.value, np.int32)
+        cells = np.zeros(n.value, np.int32)
+        disp = np.zeros((n.value, 3))
+        _check(load().lfm_halo_export(self.h, rank, C.byref(n), _ptr(keys), _ptr(cells), _ptr(disp)))
+        return keys, cells, disp
+
+    def halo_import(self, rank: int, msg):
+        keys, cells, disp = (np.ascontiguousarray(msg[0], np.int32), np.ascontiguousarray(msg[1], np.int32),
+                             np.ascontiguousarray(msg[2], np.float64))
+        _check(load().lfm_halo_import(self.h, rank, len(keys), _ptr(keys), _ptr(cells), _ptr(disp)))
+
+    def halo_sizes(self, rank: int, kind: int):
+        sb, rb = C.c_int64(), C.c_int64()
+        _check(load().lfm_halo_sizes(self.h, rank, kind, C.byref(sb), C.byref(rb)))
+        return sb.value, rb.value
+
+    def halo_pack(self, rank: int, kind: int, dev_ptr: int):
+        _check(load().lfm_halo_pack(self.h, rank, kind, dev_ptr))
+
+    def halo_unpack(self, rank: int, kind: int, dev_ptr: int):
+        _check(load().lfm_halo_unpack(self.h, rank, kind, dev_ptr))
+
+
+def connect(parts: Sequence["GpuSolver"]):
+    """Exchange the halo set-up messages between the partitions of one case
+    (the reference does this over MPI at start-up)."""
+    for q, sq in enumerate(parts):
+        for r in sq.halo_neighbours():
+            sq.halo_import(r, parts[r].halo_export(q))
+
+
+class PartGroup:
+    """Partitions of one case resident on this device, stepped together with
+    device-side halo exchange (lfm_part_group_*)."""
+
+    def __init__(self, parts: Sequence[GpuSolver]):
+        arr = (C.c_void_p * len(parts))(*[p.h for p in parts])
+        h = C.c_void_p()
+        _check(load().lfm_part_group_create(arr, len(parts), C.byref(h)))
+        self.h = h.value
+        self.parts = list(parts)
+
+    def run(self, nsteps: int):
+        _check(load().lfm_part_group_run(self.h, nsteps))
+
+    def synchronize(self):
+        _check(load().lfm_part_group_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().lfm_part_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -203,3 +203,15 @@ def cylinder_3d(nr=64, nth=64, nz=16, mach=0.2, Re=3900.0, pert=0.01, seed=303, 
     prm["dt"] = cfl * min(h_min, 0.5 * 2 * math.pi / nth, span / nz) / (U_inf + c_inf)
     del r
     return Case(f"cyl3d_{nr}x{nth}x{nz}", mesh, prm, p, T, U, alpha)
+
+
+def partition(case: Case, cell_rank) -> List[Case]:
+    """The case decomposed like decomposePar (meshgen.decompose): one Case
+    per rank with its local mesh (processor patches, face keys) and the
+    initial fields of its cells."""
+    out = []
+    for lm, cells in meshgen.decompose(case.mesh, cell_rank):
+        alpha = None if case.alpha is None else case.alpha[cells]
+        out.append(Case(f"{case.name}_rank{lm.rank}", lm, case.params, case.p[cells], case.T[cells], case.U[cells],
+                        alpha))
+    return out

@@ -90,6 +90,7 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
         if (!hs[i] || !hs[i]->impl) return fail(LFM_EINVAL, "null solver in batch");
         Solver *s = hs[i]->impl.get();
         if (!s->fp32) return fail(LFM_EINVAL, "batched launches need FP32-perturbation solvers");
+        if (!s->L.halo.empty()) return fail(LFM_EINVAL, "a partition with processor patches cannot join a batch");
         if (s->prm.have_res || s->prm.adjust_dt)
             return fail(LFM_EINVAL, "batched solvers run constant dt without residual output");
         if (i == 0) {

@@ -26,6 +26,7 @@ namespace lfm {
 
 struct KArgs32 {
     int nc, nsm0, ng, nf, ntot;
+    int ng_bnd;  // boundary ghosts [nc, nc+ng_bnd); halo ghosts after them
     const int32_t *fP, *fN;
     const float *S, *d, *w;
     const int32_t *g_off, *g_ent, *s_off, *s_ent;
@@ -293,7 +294,8 @@ template <int D, int SOLVER>
 __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) {
     if (f >= a.nf) return;
     const int P = a.fP[f], N = a.fN[f];
-    const bool ghost = N >= a.nc;
+    // is_ghost: boundary ghosts only; halo ghosts carry the neighbour's tensors
+    const bool ghost = (unsigned)(N - a.nc) < (unsigned)a.ng_bnd;
     const float w = a.w[f];
     const size_t nt = a.ntot;
     float S[D], d[D], qP[D + 2], qN[D + 2];

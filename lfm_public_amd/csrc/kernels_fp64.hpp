@@ -22,6 +22,7 @@ namespace lfm {
 
 struct KArgs {
     int nc, nsm0, ng, nf, ntot;
+    int ng_bnd;  // boundary ghosts [nc, nc+ng_bnd); halo ghosts after them
     const int32_t *fP, *fN;
     const double *S, *d, *w;        // S, d: [D][nf]
     const int32_t *g_off, *g_ent;   // gather lists
@@ -328,7 +329,8 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
     const int f = LFM_TID(blockIdx.x);
     if (f >= a.nf) return;
     const int P = a.fP[f], N = a.fN[f];
-    const bool ghost = N >= a.nc;
+    // is_ghost: boundary ghosts only; halo ghosts carry the neighbour's tensors
+    const bool ghost = (unsigned)(N - a.nc) < (unsigned)a.ng_bnd;
     const double w = a.w[f];
     const size_t nt = a.ntot;
     double S[D], d[D], qP[D + 2], qN[D + 2];

@@ -12,6 +12,8 @@
 //   src/cfd_v0.cpp:760-861                          S, d, linear weight, 1/V
 //   src/cfd_v0.cpp:278-339                          valid-first face order
 //   src/cfd_v0.cpp:867-967                          sponge sigma, initial conservatives
+//   src/cfd_v0.cpp:432-540, 587-700                 MPI ghosts, send lists
+//   src/mesh_reader.cpp:390-612                     processor-face neighbour centres
 // Geometry follows OpenFOAM's primitiveMesh face/cell centre, area and volume
 // algorithm (the reference takes them from OpenFOAM, polyMeshReaderOF.cpp:125-135,
 // 274-281).
@@ -164,8 +166,9 @@ int rk_tables(int rk_order, double A[5], double B[5]) {
     return rk_order;
 }
 
-std::string build_layout(const lfm_polymesh &m, const lfm_params &prm, const double *p, const double *T,
-                         const double *U3, const double *alpha, SolverLayout &L) {
+std::string build_layout(const lfm_polymesh &m, const lfm_partition *part, const lfm_params &prm,
+                         const double *p, const double *T, const double *U3, const double *alpha,
+                         SolverLayout &L) {
     const int D = prm.dim;
     if (D != 2 && D != 3) return "dimension must be 2 or 3";
     if (m.n_cells <= 0 || m.n_faces <= 0) return "empty mesh";
@@ -178,7 +181,12 @@ std::string build_layout(const lfm_polymesh &m, const lfm_params &prm, const dou
         if (s < m.n_internal || n < 0 || s + n > nf) return "patch range out of bounds";
         for (int f = s; f < s + n; f++) fpatch[f] = pch;
         int role = m.patch_role[pch];
-        if (role < LFM_BC_UNSET || role > LFM_BC_CYCLIC) return "unknown patch role";
+        if (role < LFM_BC_UNSET || role > LFM_BC_PROCESSOR) return "unknown patch role";
+        if (role == LFM_BC_PROCESSOR) {
+            if (!part || !part->face_key) return "processor patch without partition data";
+            if (!m.patch_nbr || m.patch_nbr[pch] < 0 || m.patch_nbr[pch] == part->rank)
+                return "processor patch without a neighbour rank";
+        }
         if (role == LFM_BC_CYCLIC) {
             int nb = m.patch_nbr ? m.patch_nbr[pch] : -1;
             if (nb < 0 || nb >= m.n_patches || m.patch_size[nb] != n)
@@ -277,29 +285,35 @@ std::string build_layout(const lfm_polymesh &m, const lfm_params &prm, const dou
     }
     std::vector<Slot> slots(soff[nc]);
 
-    // boundary ghosts: grouped by patch, submesh-0 cells in order, slots in order
+    // ghosts: boundary patches first, then processor patches; each patch's
+    // ghosts in submesh-0 cell order, slots in order (cfd_v0.cpp:384-431)
     std::vector<int32_t> gstart(m.n_patches + 1, 0);
     {
         int acc = 0;
-        for (int pch = 0; pch < m.n_patches; pch++) {
-            gstart[pch] = acc;
-            int role = m.patch_role[pch];
-            if (role == LFM_BC_EMPTY || role == LFM_BC_CYCLIC) continue;
-            for (int ci = 0; ci < nc; ci++) {
-                int c = L.orig[ci];
-                for (int i = coff[c]; i < coff[c + 1]; i++) {
-                    int f = cfaces[i];
-                    if (is_valid(f) && fpatch[f] == pch) {
-                        if (ci >= L.nsm0) return "boundary face on an interior-submesh cell";
-                        L.ghost_cell.push_back(ci);
-                        L.ghost_role.push_back(role);
-                        acc++;
+        for (int pass = 0; pass < 2; pass++) {
+            for (int pch = 0; pch < m.n_patches; pch++) {
+                int role = m.patch_role[pch];
+                if ((role == LFM_BC_PROCESSOR) != (pass == 1)) continue;
+                gstart[pch] = acc;
+                if (role == LFM_BC_EMPTY || role == LFM_BC_CYCLIC) continue;
+                for (int ci = 0; ci < nc; ci++) {
+                    int c = L.orig[ci];
+                    for (int i = coff[c]; i < coff[c + 1]; i++) {
+                        int f = cfaces[i];
+                        if (is_valid(f) && fpatch[f] == pch) {
+                            if (ci >= L.nsm0) return "boundary face on an interior-submesh cell";
+                            L.ghost_cell.push_back(ci);
+                            L.ghost_role.push_back(role);
+                            acc++;
+                        }
                     }
                 }
             }
+            if (pass == 0) L.ng_bnd = acc;
         }
         L.ng = acc;
     }
+    L.rank = part ? part->rank : -1;
     std::vector<int32_t> gcount(m.n_patches + 1, 0);
     for (int ci = 0; ci < nc; ci++) {
         const int c = L.orig[ci];
@@ -324,6 +338,24 @@ std::string build_layout(const lfm_polymesh &m, const lfm_params &prm, const dou
                 for (int k = 0; k < D; k++) sl.d[k] = 2.0 * std::fabs(dPS) * sl.S[k] / S2;
                 const int pch = fpatch[f];
                 sl.nb = nc + gstart[pch] + gcount[pch]++;
+                if (m.patch_role[pch] == LFM_BC_PROCESSOR) {
+                    // halo ghost; d and w are replaced by halo_import
+                    const int rank = m.patch_nbr[pch];
+                    size_t ni = 0;
+                    while (ni < L.halo.size() && L.halo[ni].rank != rank) ni++;
+                    if (ni == L.halo.size()) {
+                        L.halo.emplace_back();
+                        L.halo.back().rank = rank;
+                    }
+                    HaloNbr &h = L.halo[ni];
+                    h.ghost.push_back(sl.nb - nc);
+                    h.key.push_back(part->face_key[f - m.n_internal]);
+                    h.cell.push_back(ci);
+                    for (int k = 0; k < 3; k++) {
+                        h.xf.push_back(comp(fc[f], k));
+                        h.xc.push_back(comp(cc[c], k));
+                    }
+                }
             } else {
                 double vn[3];
                 for (int k = 0; k < D; k++) vn[k] = comp(cc[nb], k) - comp(cc[own], k);
@@ -434,8 +466,29 @@ std::string build_layout(const lfm_polymesh &m, const lfm_params &prm, const dou
         std::fill(cnt.begin(), cnt.end(), 0);
         for (int f = 0; f < NF; f++) {
             int P = L.fP[f], N = L.fN[f];
-            L.g_ent[L.g_off[P] + cnt[P]++] = N < nc ? f : (f | kGhostFaceFlag);
+            L.g_ent[L.g_off[P] + cnt[P]++] = (N < nc || N >= nc + L.ng_bnd) ? f : (f | kGhostFaceFlag);
             if (N < nc) L.g_ent[L.g_off[N] + cnt[N]++] = ~f;
+        }
+    }
+
+    // halo: layout face of each processor ghost, ascending send lists
+    if (!L.halo.empty()) {
+        std::vector<int32_t> face_of_ghost(L.ng, -1);
+        for (int f = 0; f < NF; f++)
+            if (L.fN[f] >= nc) face_of_ghost[L.fN[f] - nc] = f;
+        for (HaloNbr &h : L.halo) {
+            h.face.resize(h.ghost.size());
+            for (size_t j = 0; j < h.ghost.size(); j++) {
+                h.face[j] = face_of_ghost[h.ghost[j]];
+                if (h.face[j] < 0) return "internal error: processor ghost without a face";
+            }
+            h.send_cells = h.cell;
+            std::sort(h.send_cells.begin(), h.send_cells.end());
+            h.send_cells.erase(std::unique(h.send_cells.begin(), h.send_cells.end()), h.send_cells.end());
+            std::vector<int32_t> keys = h.key;
+            std::sort(keys.begin(), keys.end());
+            if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
+                return "duplicate face key on a processor patch";
         }
     }
 
@@ -474,6 +527,49 @@ std::string build_layout(const lfm_polymesh &m, const lfm_params &prm, const dou
         for (int k = 0; k < D; k++) L.q0[(size_t)(k + 1) * nc + ci] = rho * U3[3 * c + k];
         L.q0[(size_t)(D + 1) * nc + ci] = rho * E;
     }
+    return "";
+}
+
+std::string halo_import(SolverLayout &L, int ni, int n, const int32_t *key, const int32_t *cell,
+                        const double *disp) {
+    if (ni < 0 || ni >= (int)L.halo.size()) return "unknown neighbour";
+    HaloNbr &h = L.halo[ni];
+    const int nh = (int)h.key.size();
+    if (n != nh) return "set-up message size differs from the processor patch";
+    std::vector<std::pair<int32_t, int32_t>> idx(n);
+    for (int i = 0; i < n; i++) idx[i] = {key[i], i};
+    std::sort(idx.begin(), idx.end());
+    for (int i = 1; i < n; i++)
+        if (idx[i].first == idx[i - 1].first) return "duplicate face key in the set-up message";
+    // the neighbour's send list: its processor-face cells, ascending
+    std::vector<int32_t> rsend(cell, cell + n);
+    std::sort(rsend.begin(), rsend.end());
+    rsend.erase(std::unique(rsend.begin(), rsend.end()), rsend.end());
+    const int D = L.D, NF = L.nf;
+    h.recv_src.assign(nh, -1);
+    for (int j = 0; j < nh; j++) {
+        auto it = std::lower_bound(idx.begin(), idx.end(), std::make_pair(h.key[j], (int32_t)INT32_MIN));
+        if (it == idx.end() || it->first != h.key[j]) return "face key missing from the set-up message";
+        const int i = it->second;
+        h.recv_src[j] = (int32_t)(std::lower_bound(rsend.begin(), rsend.end(), cell[i]) - rsend.begin());
+        // d = (remote centre - face centre) + face centre - cell centre, owner side
+        const int f = h.face[j];
+        double d[3], dP[3], S[3];
+        for (int k = 0; k < D; k++) {
+            d[k] = disp[3 * i + k] + h.xf[3 * j + k] - h.xc[3 * j + k];
+            dP[k] = h.xc[3 * j + k] - h.xf[3 * j + k];
+            S[k] = L.S[(size_t)k * NF + f];
+        }
+        double wn = 0.0, dv = 0.0;
+        for (int k = 0; k < D; k++) {
+            const double dN = dP[k] + d[k];
+            wn += S[k] * dN;
+            dv += (-S[k] * dP[k] + S[k] * dN);
+        }
+        for (int k = 0; k < D; k++) L.d[(size_t)k * NF + f] = d[k];
+        L.w[f] = wn / dv;
+    }
+    h.imported = true;
     return "";
 }
 

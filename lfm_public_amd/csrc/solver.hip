@@ -89,6 +89,7 @@ static int create_fp64(Solver *s) {
     a.nsm0 = L.nsm0;
     a.ng = L.ng;
     a.nf = L.nf;
+    a.ng_bnd = L.ng_bnd;
     a.ntot = (int)ntot;
     int rc = 0;
     rc |= dev_upload(s, (int32_t **)&a.fP, L.fP.data(), nf);
@@ -276,36 +277,49 @@ void Solver::prof_mark(int grp, bool start) {
     prof_n++;
 }
 
-// one full time step (Mesh::solve loop body), no host synchronisation
-void Solver::step() {
-    const bool res = prm.have_res != 0;
-    for (int rk = 0; rk < nstages; rk++) {
-        // the update kernel refreshes the ghosts; a full fill is needed only
-        // after the state was set from outside
-        if (ghosts_stale) {
-            prof_mark(0, true);
-            bc();
-            prof_mark(0, false);
-            ghosts_stale = false;
-        }
-        prof_mark(1, true);
-        gradvis();
-        prof_mark(1, false);
-        prof_mark(2, true);
-        flux();
-        prof_mark(2, false);
-        prof_mark(3, true);
-        update(A[rk], B[rk], rk == 0, res && rk == 0, false);
-        prof_mark(3, false);
+void Solver::stage_a(int rk) {
+    (void)rk;
+    // the update kernel refreshes the boundary ghosts; a full fill is needed
+    // only after the state was set from outside
+    if (ghosts_stale) {
+        prof_mark(0, true);
+        bc();
+        prof_mark(0, false);
+        ghosts_stale = false;
     }
+    prof_mark(1, true);
+    gradvis();
+    prof_mark(1, false);
+}
+
+void Solver::stage_b(int rk) {
+    const bool res = prm.have_res != 0;
+    prof_mark(2, true);
+    flux();
+    prof_mark(2, false);
+    prof_mark(3, true);
+    update(A[rk], B[rk], rk == 0, res && rk == 0, false);
+    prof_mark(3, false);
+}
+
+void Solver::end_step(bool own_dt) {
     prof_mark(4, true);
     if (prm.adjust_dt)
-        cfl_dt(0, prm.max_co, d_dtmin, d_dt_ptr());
+        cfl_dt(0, prm.max_co, d_dtmin, own_dt ? d_dt_ptr() : nullptr);
     else
         cfl_dt(1, dt, d_cfl, nullptr);
     prof_mark(4, false);
     steps_done++;
-    if (res) res_valid = true;
+    if (prm.have_res) res_valid = true;
+}
+
+// one full time step (Mesh::solve loop body), no host synchronisation
+void Solver::step() {
+    for (int rk = 0; rk < nstages; rk++) {
+        stage_a(rk);
+        stage_b(rk);
+    }
+    end_step(true);
 }
 
 }  // namespace lfm
@@ -323,6 +337,7 @@ int lfm_patch_role(const char *type, const char *name) {
     const std::string t(type), n(name);
     if (t == "empty") return LFM_BC_EMPTY;
     if (t == "cyclic") return LFM_BC_CYCLIC;
+    if (t == "processor") return LFM_BC_PROCESSOR;
     if (t == "wall") return LFM_BC_WALL;
     if (t == "patch") {
         if (n == "inlet" || n == "Inlet" || n == "Inflow" || n == "inflow") return LFM_BC_INLET;
@@ -333,6 +348,12 @@ int lfm_patch_role(const char *type, const char *name) {
 
 int lfm_solver_create(const lfm_polymesh *mesh, const lfm_params *prm, const double *p, const double *T,
                       const double *U3, const double *alpha, lfm_solver **out) {
+    return lfm_solver_create_part(mesh, nullptr, prm, p, T, U3, alpha, out);
+}
+
+int lfm_solver_create_part(const lfm_polymesh *mesh, const lfm_partition *part, const lfm_params *prm,
+                           const double *p, const double *T, const double *U3, const double *alpha,
+                           lfm_solver **out) {
     if (!mesh || !prm || !p || !T || !U3 || !out) return fail(LFM_EINVAL, "null argument");
     *out = nullptr;
     if (prm->solver < 0 || prm->solver > 2) return fail(LFM_EINVAL, "solver must be 0 (M1), 1 (M2) or 2 (M2AUSM)");
@@ -343,7 +364,7 @@ int lfm_solver_create(const lfm_polymesh *mesh, const lfm_params *prm, const dou
         return fail(LFM_EINVAL, "only laminar simulationType is supported (calc_VIS_Smagorinsky not built)");
     auto s = std::make_unique<Solver>();
     s->prm = *prm;
-    std::string err = lfm::build_layout(*mesh, *prm, p, T, U3, alpha, s->L);
+    std::string err = lfm::build_layout(*mesh, part, *prm, p, T, U3, alpha, s->L);
     if (!err.empty()) return fail(LFM_EINVAL, err);
     s->gas = lfm::derive_gas(*prm);
     s->nstages = lfm::rk_tables(prm->rk_order, s->A, s->B);
@@ -356,6 +377,19 @@ int lfm_solver_create(const lfm_polymesh *mesh, const lfm_params *prm, const dou
     HIP_TRY(hipHostMalloc((void **)&s->h_scalar, 64 * sizeof(double), hipHostMallocDefault));
     int rc = s->fp32 ? lfm::create_fp32p(s.get()) : lfm::create_fp64(s.get());
     if (rc) return rc;
+    s->halo_dev.resize(s->L.halo.size());
+    for (size_t i = 0; i < s->L.halo.size(); i++) {
+        const auto &hn = s->L.halo[i];
+        auto &hd = s->halo_dev[i];
+        hd.n_send = (int)hn.send_cells.size();
+        hd.n_recv = (int)hn.ghost.size();
+        std::vector<int32_t> gslot(hn.ghost.size());
+        for (size_t j = 0; j < hn.ghost.size(); j++) gslot[j] = s->L.nc + hn.ghost[j];
+        if (lfm::dev_upload(s.get(), &hd.send_cells, hn.send_cells.data(), hn.send_cells.size()) ||
+            lfm::dev_upload(s.get(), &hd.recv_ghost, gslot.data(), gslot.size()) ||
+            lfm::dev_zeros(s.get(), &hd.recv_src, gslot.size()))
+            return LFM_EDEVICE;
+    }
     HIP_TRY(hipDeviceSynchronize());
     auto *h = new lfm_solver;
     h->impl = std::move(s);
@@ -424,6 +458,7 @@ int lfm_calc_gradients(lfm_solver *h) {
 
 int lfm_calc_vis(lfm_solver *h) {
     if (!h) return fail(LFM_EINVAL, "null handle");
+    if (!h->impl->halo_ready()) return fail(LFM_ESTATE, "halo set-up not imported for every neighbour");
     h->impl->gradvis();
     HIP_TRY(hipGetLastError());
     return LFM_OK;
@@ -433,6 +468,7 @@ int lfm_one_rk_step(lfm_solver *h, int rk, double dt, double *res) {
     if (!h) return fail(LFM_EINVAL, "null handle");
     Solver *s = h->impl.get();
     if (rk < 0 || rk > 4) return fail(LFM_EINVAL, "rk stage out of range");
+    if (!s->halo_ready()) return fail(LFM_ESTATE, "halo set-up not imported for every neighbour");
     const double B = s->B[rk];
     s->h_scalar[0] = dt;
     HIP_TRY(hipMemcpyAsync(s->d_dt_ptr(), s->h_scalar, sizeof(double), hipMemcpyHostToDevice, s->stream));
@@ -477,6 +513,9 @@ int lfm_compute_cfl(lfm_solver *h, double dt, double *cfl_out) {
 int lfm_run(lfm_solver *h, int nsteps) {
     if (!h || nsteps < 0) return fail(LFM_EINVAL, "bad argument");
     Solver *s = h->impl.get();
+    if (!s->L.halo.empty())
+        return fail(LFM_ESTATE, "a partition with processor patches steps through lfm_part_group_run or the "
+                                "per-stage calls with halo exchange");
     for (int i = 0; i < nsteps; i++) s->step();
     HIP_TRY(hipGetLastError());
     return LFM_OK;
@@ -491,6 +530,9 @@ int lfm_synchronize(lfm_solver *h) {
 int lfm_time_steps(lfm_solver *h, int nsteps, double *ms_out) {
     if (!h || !ms_out || nsteps < 0) return fail(LFM_EINVAL, "bad argument");
     Solver *s = h->impl.get();
+    if (!s->L.halo.empty())
+        return fail(LFM_ESTATE, "a partition with processor patches steps through lfm_part_group_run or the "
+                                "per-stage calls with halo exchange");
     HIP_TRY(hipEventRecord(s->ev0, s->stream));
     for (int i = 0; i < nsteps; i++) s->step();
     HIP_TRY(hipEventRecord(s->ev1, s->stream));
@@ -504,6 +546,9 @@ int lfm_time_steps(lfm_solver *h, int nsteps, double *ms_out) {
 int lfm_profile_steps(lfm_solver *h, int nsteps, double *ms_per_group, double *ms_total) {
     if (!h || !ms_per_group || !ms_total || nsteps < 1) return fail(LFM_EINVAL, "bad argument");
     Solver *s = h->impl.get();
+    if (!s->L.halo.empty())
+        return fail(LFM_ESTATE, "a partition with processor patches steps through lfm_part_group_run or the "
+                                "per-stage calls with halo exchange");
     const size_t need = (size_t)nsteps * (s->nstages * 8 + 2);
     for (hipEvent_t e : s->prof_ev) (void)hipEventDestroy(e);
     s->prof_ev.assign(need, nullptr);

@@ -53,6 +53,7 @@ int create_fp32p(Solver *s) {
     a.nsm0 = L.nsm0;
     a.ng = L.ng;
     a.nf = L.nf;
+    a.ng_bnd = L.ng_bnd;
     a.ntot = (int)ntot;
     const Gas &g = s->gas;
     // freestream constants (FP64 derivations, rounded once)

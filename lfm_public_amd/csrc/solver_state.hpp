@@ -56,6 +56,31 @@ struct Solver {
     size_t prof_n = 0;
     void prof_mark(int grp, bool start);
 
+    // halo lists on the device, one entry per L.halo neighbour
+    struct HaloDev {
+        int32_t *send_cells = nullptr;  // [n_send] local cells, ascending
+        int32_t *recv_ghost = nullptr;  // [n_recv] ghost slot (combined index nc+g)
+        int32_t *recv_src = nullptr;    // [n_recv] record in the received buffer
+        int n_send = 0, n_recv = 0;
+    };
+    std::vector<HaloDev> halo_dev;
+    bool halo_ready() const {
+        for (const HaloNbr &h : L.halo)
+            if (!h.imported) return false;
+        return true;
+    }
+    int halo_index(int rank) const {
+        for (size_t i = 0; i < L.halo.size(); i++)
+            if (L.halo[i].rank == rank) return (int)i;
+        return -1;
+    }
+    int halo_record(int kind) const {
+        return ((kind & 1) ? L.D + 2 : 0) + ((kind & 2) ? 2 * L.D * L.D + 2 * L.D : 0);
+    }
+    int halo_pack(int ni, int kind, void *buf);
+    int halo_unpack(int ni, int kind, const void *buf);
+    int halo_upload_geometry();  // d, w of the faces after an import
+
     bool ghosts_stale = true;  // ghost states not derived from the current q
     double pending_A = 1.0;
     int pending_rk = 0;
@@ -71,6 +96,12 @@ struct Solver {
     void update(double A, double B, bool zero_dq, bool res, bool accumulate_res);
     void cfl_dt(int mode, double param, double *out, double *dt_dst);
     void step();
+    // pieces of step(): stage part A (ghost fill if stale, gradients and
+    // viscous tensors), part B (fluxes, update), and the end-of-step CFL/dt
+    // reduction (dt written to the own deltaT when own_dt)
+    void stage_a(int rk);
+    void stage_b(int rk);
+    void end_step(bool own_dt);
     void zero_state_for_step();
 
     int get_field(int which, double *out);

@@ -23,7 +23,7 @@ import numpy as np
 
 # Boundary roles, resolved from patch type and name with the rule of
 # init_boundary_conditions (src/cfd_v0.cpp:973-1005).
-BC_UNSET, BC_WALL, BC_INLET, BC_OUTLET, BC_EMPTY, BC_CYCLIC = 0, 1, 2, 3, 4, 5
+BC_UNSET, BC_WALL, BC_INLET, BC_OUTLET, BC_EMPTY, BC_CYCLIC, BC_PROCESSOR = 0, 1, 2, 3, 4, 5, 6
 INLET_NAMES = ("inlet", "Inlet", "Inflow", "inflow")
 OUTLET_NAMES = ("outlet", "Outlet", "Outflow", "outflow")
 
@@ -34,6 +34,8 @@ def patch_role(ptype: str, name: str) -> int:
         return BC_EMPTY
     if ptype == "cyclic":
         return BC_CYCLIC
+    if ptype == "processor":
+        return BC_PROCESSOR
     if ptype == "wall":
         return BC_WALL
     if ptype == "patch":
@@ -51,6 +53,7 @@ class Patch:
     start: int
     size: int
     neighbour: Optional[str] = None  # cyclic partner
+    neighb_proc: int = -1            # processor patch: neighbour rank
 
 
 @dataclasses.dataclass
@@ -63,6 +66,10 @@ class PolyMesh:
     n_cells: int
     patches: List[Patch]
     dim: int = 2
+    # decomposed case: this partition's rank and, per boundary face, the
+    # global face index shared by both sides of a processor face (-1 else)
+    rank: int = -1
+    face_key: Optional[np.ndarray] = None
 
     @property
     def n_faces(self) -> int:
@@ -77,7 +84,8 @@ class PolyMesh:
         start = np.array([p.start for p in self.patches], np.int32)
         size = np.array([p.size for p in self.patches], np.int32)
         role = np.array([patch_role(p.type, p.name) for p in self.patches], np.int32)
-        nbr = np.array([names.index(p.neighbour) if p.neighbour else -1 for p in self.patches], np.int32)
+        nbr = np.array([p.neighb_proc if p.type == "processor" else (names.index(p.neighbour) if p.neighbour else -1)
+                        for p in self.patches], np.int32)
         return start, size, role, nbr
 
     def face_points(self, f: int) -> np.ndarray:
@@ -434,3 +442,103 @@ def cell_centres_approx(m: PolyMesh) -> np.ndarray:
     np.add.at(cc, m.neighbour, fc[:m.n_internal])
     np.add.at(nfc, m.neighbour, 1)
     return cc / nfc[:, None]
+
+
+def decompose(mesh: PolyMesh, cell_rank: np.ndarray):
+    """Split a case into partitions the way decomposePar lays them out.
+
+    Returns, per rank, ``(local mesh, cell addressing)``.  Local cells keep
+    their global order; internal faces keep theirs; every original patch is
+    kept (possibly empty), restricted to the rank's faces; then one
+    ``processor`` patch per neighbour rank (ascending), faces in global face
+    order, owned by the local cell (the higher rank sees the face reversed).
+    ``face_key`` gives each processor face its global face index.  A cyclic
+    face and its partner must fall on the same rank.
+    """
+    cell_rank = np.asarray(cell_rank, np.int64)
+    assert cell_rank.shape[0] == mesh.n_cells
+    nr = int(cell_rank.max()) + 1
+    nf, ni = mesh.n_faces, mesh.n_internal
+    owner = mesh.owner.astype(np.int64)
+    nbr = mesh.neighbour.astype(np.int64)
+    ro = cell_rank[owner]
+    rn = cell_rank[nbr]
+    start, size, role, pnbr = mesh.patch_arrays()
+    fpatch = np.full(nf, -1, np.int64)
+    for i, p in enumerate(mesh.patches):
+        fpatch[p.start:p.start + p.size] = i
+    for i, p in enumerate(mesh.patches):
+        if role[i] == BC_CYCLIC:
+            tw = np.arange(p.size) + int(start[pnbr[i]])
+            if np.any(ro[p.start:p.start + p.size] != ro[tw]):
+                raise ValueError(f"cyclic patch {p.name} straddles partitions")
+    out = []
+    for r in range(nr):
+        cells = np.nonzero(cell_rank == r)[0]
+        g2l = np.full(mesh.n_cells, -1, np.int64)
+        g2l[cells] = np.arange(cells.shape[0])
+        fint = np.nonzero((ro[:ni] == r) & (rn == r))[0]
+        bpat = []  # (name, type, faces, neighbour name, neighb_proc)
+        for i, p in enumerate(mesh.patches):
+            fs = np.arange(p.start, p.start + p.size)
+            bpat.append((p.name, p.type, fs[ro[fs] == r], p.neighbour, -1, False))
+        cut = np.nonzero(ro[:ni] != rn)[0]
+        nbrs = sorted(set(rn[cut][ro[cut] == r].tolist()) | set(ro[cut][rn[cut] == r].tolist()))
+        for q in nbrs:
+            fs = cut[((ro[cut] == r) & (rn[cut] == q)) | ((rn[cut] == r) & (ro[cut] == q))]
+            bpat.append((f"procBoundary{r}to{q}", "processor", np.sort(fs), None, q, True))
+        allf = [fint] + [b[2] for b in bpat]
+        faces = np.concatenate(allf)
+        # local owner / neighbour and point lists (processor faces seen from r)
+        lown = np.empty(faces.shape[0], np.int64)
+        pts_list = []
+        k = 0
+        flip_all = np.zeros(faces.shape[0], bool)
+        for idx, fs in enumerate(allf):
+            is_proc = idx > 0 and bpat[idx - 1][5]
+            for f in fs:
+                if is_proc and int(ro[f]) != r:
+                    lown[k] = g2l[nbr[f]]
+                    flip_all[k] = True
+                else:
+                    lown[k] = g2l[owner[f]]
+                k += 1
+        lnbr = g2l[nbr[fint]]
+        used = np.zeros(mesh.points.shape[0], bool)
+        for f in faces:
+            used[mesh.face_points(int(f))] = True
+        pmap = np.full(mesh.points.shape[0], -1, np.int64)
+        pmap[used] = np.arange(int(used.sum()))
+        offs = [0]
+        flat = []
+        for j, f in enumerate(faces):
+            fp = pmap[mesh.face_points(int(f))]
+            if flip_all[j]:
+                fp = fp[::-1]
+            flat.append(fp)
+            offs.append(offs[-1] + fp.shape[0])
+        patches = []
+        s0 = fint.shape[0]
+        for (name, ptype, fs, pn, q, is_proc) in bpat:
+            patches.append(Patch(name, ptype, s0, int(fs.shape[0]), pn, q))
+            s0 += int(fs.shape[0])
+        key = np.full(faces.shape[0] - fint.shape[0], -1, np.int32)
+        off = fint.shape[0]
+        for (name, ptype, fs, pn, q, is_proc) in bpat:
+            if is_proc:
+                key[off - fint.shape[0]:off - fint.shape[0] + fs.shape[0]] = fs
+            off += fs.shape[0]
+        lm = PolyMesh(np.ascontiguousarray(mesh.points[used]), np.asarray(offs, np.int32),
+                      np.concatenate(flat).astype(np.int32), lown.astype(np.int32), lnbr.astype(np.int32),
+                      int(cells.shape[0]), patches, mesh.dim, rank=r, face_key=key)
+        out.append((lm, cells))
+    return out
+
+
+def slab_ranks(mesh: PolyMesh, n: int, axis: int = 0) -> np.ndarray:
+    """Cell -> rank by equal-count slabs along a coordinate axis."""
+    cc = cell_centres_approx(mesh)[:, axis]
+    order = np.argsort(cc, kind="stable")
+    rank = np.empty(mesh.n_cells, np.int64)
+    rank[order] = np.arange(mesh.n_cells) * n // mesh.n_cells
+    return rank

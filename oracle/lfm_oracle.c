@@ -86,6 +86,13 @@ typedef struct {
     double gamma, gm1, R, R_inv, pr_inv, cp, mu;
     double p_inf, T_inf, U_inf[3], rho_inf, E_inf;
     double RESsum[5];
+    /* processor faces (grouped by neighbour, first-appearance order) and
+     * MPI ghosts v[nc + ng + ...] (one per distinct neighbour cell) */
+    int nproc, nnb;
+    int *pf_cell, *pf_slot, *pf_key;
+    double *pf_xf, *pf_xc;
+    int nb_rank[64], nb_off[65];
+    int *nb_recv[64], nb_nrecv[64], nb_imported[64];
 } oracle;
 
 /* ------------------------------------------------------------------ */
@@ -309,7 +316,7 @@ void *lfmo_create(const lfmo_mesh *m, const lfmo_params *prm,
             int f = cfaces[i];
             if (!valid[f]) continue;
             int pch = fpatch[f];
-            if (pch >= 0 && m->patch_role[pch] != LFMO_BC_CYCLIC) {
+            if (pch >= 0 && m->patch_role[pch] != LFMO_BC_CYCLIC && m->patch_role[pch] != LFMO_BC_PROCESSOR) {
                 if (sm[c] != 0) {
                     seterr(err, errlen, "boundary face on a submesh-1 cell");
                     goto fail;
@@ -318,7 +325,45 @@ void *lfmo_create(const lfmo_mesh *m, const lfmo_params *prm,
             }
         }
     o->ng = ng;
-    o->v = calloc((size_t)(nc + ng), sizeof(solvars));
+    /* processor faces: grouped by neighbour rank in first-appearance order
+     * over (submesh-0 cell, face) -- allocate_ghost_cells, cfd_v0.cpp:437-462 */
+    int nproc = 0;
+    for (int ci = 0; ci < nsm0; ci++) {
+        int c = o->orig[ci];
+        for (int i = coff[c]; i < coff[c + 1]; i++) {
+            int f = cfaces[i];
+            int pch = fpatch[f];
+            if (!valid[f] || pch < 0 || m->patch_role[pch] != LFMO_BC_PROCESSOR) continue;
+            int r = m->patch_nbr[pch], b = 0;
+            while (b < o->nnb && o->nb_rank[b] != r) b++;
+            if (b == o->nnb) {
+                if (o->nnb == 64) { seterr(err, errlen, "more than 64 neighbour ranks"); goto fail; }
+                o->nb_rank[o->nnb++] = r;
+                o->nb_off[b + 1] = 0;
+            }
+            o->nb_off[b + 1]++;
+            nproc++;
+        }
+    }
+    for (int c = 0; c < nc; c++)
+        if (sm[c] != 0)
+            for (int i = coff[c]; i < coff[c + 1]; i++) {
+                int pch = fpatch[cfaces[i]];
+                if (pch >= 0 && m->patch_role[pch] == LFMO_BC_PROCESSOR) {
+                    seterr(err, errlen, "processor face on a submesh-1 cell");
+                    goto fail;
+                }
+            }
+    if (nproc && (!m->face_key || m->rank < 0)) { seterr(err, errlen, "processor patch without partition data"); goto fail; }
+    o->nb_off[0] = 0;
+    for (int b = 0; b < o->nnb; b++) o->nb_off[b + 1] += o->nb_off[b];
+    o->nproc = nproc;
+    o->pf_cell = malloc(sizeof(int) * (nproc + 1));
+    o->pf_slot = malloc(sizeof(int) * (nproc + 1));
+    o->pf_key = malloc(sizeof(int) * (nproc + 1));
+    o->pf_xf = malloc(sizeof(double) * 3 * (nproc + 1));
+    o->pf_xc = malloc(sizeof(double) * 3 * (nproc + 1));
+    o->v = calloc((size_t)(nc + ng + nproc), sizeof(solvars));
     o->g = calloc((size_t)nc, sizeof(cellgeo));
     o->ghost_cell = malloc(sizeof(int) * (ng + 1));
     o->ghost_role = malloc(sizeof(int) * (ng + 1));
@@ -336,7 +381,8 @@ void *lfmo_create(const lfmo_mesh *m, const lfmo_params *prm,
     {
         int gi = 0;
         for (int pch = 0; pch < m->n_patches; pch++) {
-            if (m->patch_role[pch] == LFMO_BC_EMPTY || m->patch_role[pch] == LFMO_BC_CYCLIC) continue;
+            if (m->patch_role[pch] == LFMO_BC_EMPTY || m->patch_role[pch] == LFMO_BC_CYCLIC ||
+                m->patch_role[pch] == LFMO_BC_PROCESSOR) continue;
             for (int ci = 0; ci < nsm0; ci++) {
                 int c = o->orig[ci];
                 for (int i = coff[c]; i < coff[c + 1]; i++) {
@@ -360,13 +406,15 @@ void *lfmo_create(const lfmo_mesh *m, const lfmo_params *prm,
         int acc = 0;
         for (int pch = 0; pch < m->n_patches; pch++) {
             gstart[pch] = acc;
-            if (m->patch_role[pch] == LFMO_BC_EMPTY || m->patch_role[pch] == LFMO_BC_CYCLIC) continue;
+            if (m->patch_role[pch] == LFMO_BC_EMPTY || m->patch_role[pch] == LFMO_BC_CYCLIC ||
+                m->patch_role[pch] == LFMO_BC_PROCESSOR) continue;
             for (int ci = 0; ci < nsm0; ci++) {
                 int c = o->orig[ci];
                 for (int i = coff[c]; i < coff[c + 1]; i++)
                     if (valid[cfaces[i]] && fpatch[cfaces[i]] == pch) acc++;
             }
         }
+        int pcount[64] = {0};
         for (int ci = 0; ci < nc; ci++) {
             int c = o->orig[ci];
             cellgeo *G = &o->g[ci];
@@ -390,7 +438,24 @@ void *lfmo_create(const lfmo_mesh *m, const lfmo_params *prm,
                 for (int k = 0; k < D; k++) G->S[s][k] = vsign * fa[3 * f + k];
                 for (int k = 0; k < D; k++) dP[s][k] = cc[3 * c + k] - fc[3 * f + k];
                 ghostslot[s] = 0;
-                if (ncell < 0) {
+                if (pch >= 0 && m->patch_role[pch] == LFMO_BC_PROCESSOR) {
+                    /* CELL_MPI: neighbour is an MPI ghost (valid in reorder_faces,
+                     * sm_id MAX_MPI_RANKS); d and w are set when the neighbour's
+                     * centres arrive (lfmo_halo_import) */
+                    int b = 0;
+                    while (o->nb_rank[b] != m->patch_nbr[pch]) b++;
+                    int k = o->nb_off[b] + pcount[b]++;
+                    o->pf_cell[k] = ci;
+                    o->pf_key[k] = m->face_key[f - m->n_internal];
+                    for (int q = 0; q < 3; q++) {
+                        o->pf_xf[3 * k + q] = fc[3 * f + q];
+                        o->pf_xc[3 * k + q] = cc[3 * c + q];
+                    }
+                    for (int q = 0; q < D; q++) G->d[s][q] = 0.0;
+                    G->neigh[s] = nc + ng + k;
+                    o->v[nc + ng + k].sm_id = GHOST_SM;
+                    o->v[nc + ng + k].id = k;
+                } else if (ncell < 0) {
                     /* CELL_NONE: mirror, cfd_v0.cpp:812-824 */
                     double S2 = 0.0, dPS = 0.0;
                     for (int k = 0; k < D; k++) S2 += G->S[s][k] * G->S[s][k];
@@ -447,6 +512,13 @@ void *lfmo_create(const lfmo_mesh *m, const lfmo_params *prm,
         T2.nvalid = vi;
         *G = T2;
     }
+
+    /* slot of each processor face after the reorder */
+    for (int ci = 0; ci < nc; ci++)
+        for (int j = 0; j < o->g[ci].nface; j++) {
+            int n = o->g[ci].neigh[j];
+            if (n >= nc + ng) o->pf_slot[n - nc - ng] = j;
+        }
 
     /* RK tables, cfd_v0.cpp:86-128 */
     switch (prm->rk_order) {
@@ -522,6 +594,8 @@ void lfmo_destroy(void *h)
 {
     oracle *o = h;
     if (!o) return;
+    free(o->pf_cell); free(o->pf_slot); free(o->pf_key); free(o->pf_xf); free(o->pf_xc);
+    for (int b = 0; b < o->nnb; b++) free(o->nb_recv[b]);
     free(o->orig); free(o->comb); free(o->v); free(o->g); free(o->ghost_cell); free(o->ghost_role);
     free(o->centre); free(o->volume);
     free(o);
@@ -1122,6 +1196,188 @@ static void run_stage(oracle *o, int rk)
     }
     calc_VIS(o);
     one_rk_step(o, rk, o->dt);
+}
+
+int lfmo_stage_phase(void *h, int rk, int begin_step, int phase)
+{
+    oracle *o = h;
+    if (phase == 0) {
+        if (begin_step) {
+            prepare_for_timestep(o);
+            for (int k = 0; k < 5; k++) o->RESsum[k] = 0.0;
+        }
+        prepare_for_RKstep(o, rk);
+        set_boundary_conditions(o);
+        if (o->minmod) {
+            if (o->solver == 2) calc_gradients_M2AUSM(o);
+            else calc_gradients(o);
+        }
+        calc_VIS(o);
+    } else {
+        one_rk_step(o, rk, o->dt);
+    }
+    return 0;
+}
+
+void lfmo_set_dt(void *h, double dt) { ((oracle *)h)->dt = dt; }
+
+/* ------------------------------------------------------------------ */
+/* partitions: set-up messages and halo payloads                       */
+/* ------------------------------------------------------------------ */
+static int nb_index(const oracle *o, int rank)
+{
+    for (int b = 0; b < o->nnb; b++)
+        if (o->nb_rank[b] == rank) return b;
+    return -1;
+}
+
+int lfmo_halo_nbrs(void *h, int *ranks)
+{
+    oracle *o = h;
+    if (ranks)
+        for (int b = 0; b < o->nnb; b++) ranks[b] = o->nb_rank[b];
+    return o->nnb;
+}
+
+/* mesh_reader.cpp:521-555: face id, owner cell, cell centre - face centre */
+int lfmo_halo_export(void *h, int rank, int *keys, int *cells, double *disp)
+{
+    oracle *o = h;
+    int b = nb_index(o, rank);
+    if (b < 0) return -1;
+    int n = o->nb_off[b + 1] - o->nb_off[b];
+    for (int i = 0; i < n; i++) {
+        int k = o->nb_off[b] + i;
+        if (keys) keys[i] = o->pf_key[k];
+        if (cells) cells[i] = o->pf_cell[k];
+        if (disp)
+            for (int q = 0; q < 3; q++) disp[3 * i + q] = o->pf_xc[3 * k + q] - o->pf_xf[3 * k + q];
+    }
+    return n;
+}
+
+static int cmp_int(const void *a, const void *b)
+{
+    int x = *(const int *)a, y = *(const int *)b;
+    return (x > y) - (x < y);
+}
+
+/* mesh_reader.cpp:577-605 (v_neigh from the received displacement),
+ * cfd_v0.cpp:800-855 (d, linear weight), cfd_v0.cpp:616-700 (the
+ * neighbour's send list = its cells on the shared faces, ascending) */
+int lfmo_halo_import(void *h, int rank, int n, const int *keys, const int *cells, const double *disp)
+{
+    oracle *o = h;
+    const int D = o->D;
+    int b = nb_index(o, rank);
+    if (b < 0 || n != o->nb_off[b + 1] - o->nb_off[b]) return -1;
+    int *uniq = malloc(sizeof(int) * (n + 1));
+    for (int i = 0; i < n; i++) uniq[i] = cells[i];
+    qsort(uniq, (size_t)n, sizeof(int), cmp_int);
+    int nu = 0;
+    for (int i = 0; i < n; i++)
+        if (nu == 0 || uniq[nu - 1] != uniq[i]) uniq[nu++] = uniq[i];
+    free(o->nb_recv[b]);
+    o->nb_recv[b] = uniq;
+    o->nb_nrecv[b] = nu;
+    const int base = o->nc + o->ng + o->nb_off[b];
+    for (int i = 0; i < nu; i++) {
+        solvars *gv = &o->v[base + i];
+        gv->sm_id = GHOST_SM;
+        gv->id = uniq[i];
+        gv->is_ghost = 0;
+        gv->vol_inv = 0.0;
+    }
+    for (int k = o->nb_off[b]; k < o->nb_off[b + 1]; k++) {
+        int i = 0;
+        while (i < n && keys[i] != o->pf_key[k]) i++;
+        if (i == n) return -2;
+        int lo = 0, hi = nu - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            if (uniq[mid] < cells[i]) lo = mid + 1; else hi = mid;
+        }
+        cellgeo *G = &o->g[o->pf_cell[k]];
+        int s = o->pf_slot[k];
+        double dP[3], dN[3], w = 0.0, div = 0.0;
+        for (int q = 0; q < D; q++) {
+            G->d[s][q] = disp[3 * i + q] + o->pf_xf[3 * k + q] - o->pf_xc[3 * k + q];
+            dP[q] = o->pf_xc[3 * k + q] - o->pf_xf[3 * k + q];
+        }
+        for (int q = 0; q < D; q++) dN[q] = dP[q] + G->d[s][q];
+        for (int q = 0; q < D; q++) {
+            w += G->S[s][q] * dN[q];
+            div += (-G->S[s][q] * dP[q] + G->S[s][q] * dN[q]);
+        }
+        G->w[s] = w / div;
+        G->neigh[s] = base + lo;
+    }
+    o->nb_imported[b] = 1;
+    return 0;
+}
+
+/* one cell's record: q, dudx, dTdx, tauMC, sigmaU (cfd_v0.cpp:3303-3331) */
+static int record(const oracle *o, solvars *v, int kind, double *buf, int dir)
+{
+    const int D = o->D;
+    int m = 0;
+#define XFER(x) do { if (dir) buf[m] = (x); else (x) = buf[m]; m++; } while (0)
+    if (kind & 1)
+        for (int i = 0; i < D + 2; i++) XFER(v->q[i]);
+    if (kind & 2) {
+        for (int i = 0; i < D; i++)
+            for (int j = 0; j < D; j++) XFER(v->dudx[i][j]);
+        for (int i = 0; i < D; i++) XFER(v->dTdx[i]);
+        for (int i = 0; i < D; i++)
+            for (int j = 0; j < D; j++) XFER(v->tauMC[i][j]);
+        for (int i = 0; i < D; i++) XFER(v->sigmaU[i]);
+    }
+    if (kind & 4) {
+        /* not a reference payload: the M2AUSM minmod gradients, which the
+         * reference never exchanges (so its partitioned runs differ from
+         * serial at processor faces); used to show that this is the only
+         * difference */
+        for (int i = 0; i < D; i++) XFER(v->rho_grad[i]);
+        for (int i = 0; i < D; i++) XFER(v->p_grad[i]);
+        for (int i = 0; i < D; i++)
+            for (int j = 0; j < D; j++) XFER(v->U_grad[i][j]);
+    }
+#undef XFER
+    return m;
+}
+
+int lfmo_halo_pack(void *h, int rank, int kind, double *buf)
+{
+    oracle *o = h;
+    int b = nb_index(o, rank);
+    if (b < 0) return -1;
+    /* send list: boundary-submesh cells with a face to this rank, in cell
+     * order, each once (assign_pointers, cfd_v0.cpp:616-639) */
+    int off = 0, last = -1;
+    int n = o->nb_off[b + 1] - o->nb_off[b];
+    int *cl = malloc(sizeof(int) * (n + 1));
+    for (int i = 0; i < n; i++) cl[i] = o->pf_cell[o->nb_off[b] + i];
+    qsort(cl, (size_t)n, sizeof(int), cmp_int);
+    for (int i = 0; i < n; i++) {
+        if (cl[i] == last) continue;
+        last = cl[i];
+        off += record(o, &o->v[cl[i]], kind, buf + off, 1);
+    }
+    free(cl);
+    return off;
+}
+
+/* unpack_mpi_data[_solvars|_viscous], cfd_v0.cpp:3608-3692 */
+int lfmo_halo_unpack(void *h, int rank, int kind, const double *buf)
+{
+    oracle *o = h;
+    int b = nb_index(o, rank);
+    if (b < 0 || !o->nb_imported[b]) return -1;
+    int off = 0;
+    const int base = o->nc + o->ng + o->nb_off[b];
+    for (int i = 0; i < o->nb_nrecv[b]; i++)
+        off += record(o, &o->v[base + i], kind, (double *)buf + off, 0);
+    return off;
 }
 
 int lfmo_stage(void *h, int rk, int begin_step)

@@ -26,7 +26,8 @@ enum {
     LFMO_BC_INLET   = 2,
     LFMO_BC_OUTLET  = 3,
     LFMO_BC_EMPTY   = 4, /* faces masked out (polyMeshReaderOF.cpp:47-59) */
-    LFMO_BC_CYCLIC  = 5  /* coupled to patch_nbr (polyMeshReaderOF.cpp:226-248) */
+    LFMO_BC_CYCLIC  = 5, /* coupled to patch_nbr (polyMeshReaderOF.cpp:226-248) */
+    LFMO_BC_PROCESSOR = 6 /* processor patch: patch_nbr is the neighbour rank */
 };
 
 typedef struct {
@@ -45,6 +46,9 @@ typedef struct {
     const int *patch_role;     /* LFMO_BC_*                         */
     const int *patch_nbr;      /* cyclic partner patch or -1        */
     const int *cell_submesh;   /* optional override, NULL = rule    */
+    const int *face_key;       /* per boundary face: global face id  */
+                               /* (processor faces), NULL if serial  */
+    int rank;                  /* this partition's rank (-1 serial)  */
 } lfmo_mesh;
 
 typedef struct {
@@ -100,6 +104,22 @@ double lfmo_compute_cfl(void *h, double dt);
 int    lfmo_get_slots(void *h, int *nface, int *nvalid, int *nbr,
                       double *S, double *d, double *w);
 int    lfmo_n_ghost(void *h);
+
+/* Partitions (src/cfd_v0.cpp:432-748 MPI ghosts and send lists;
+ * src/mesh_reader.cpp:390-612 neighbour centres; src/cfd_v0.cpp:3284-3692
+ * payloads).  Halo ghosts are one per neighbour cell, as ghost_mpi. */
+int    lfmo_halo_nbrs(void *h, int *ranks);                 /* returns n      */
+int    lfmo_halo_export(void *h, int rank, int *keys, int *cells, double *disp); /* returns n */
+int    lfmo_halo_import(void *h, int rank, int n, const int *keys, const int *cells, const double *disp);
+/* kind: 1 solvars, 2 viscous, 3 packed; bit 4 adds the M2AUSM minmod
+ * gradients (rho, p, U), which no reference payload carries; returns the
+ * number of doubles */
+int    lfmo_halo_pack(void *h, int rank, int kind, double *buf);
+int    lfmo_halo_unpack(void *h, int rank, int kind, const double *buf);
+/* Stage halves: phase 0 = prepare (+prepare_for_timestep when begin_step),
+ * boundary conditions, gradients, calc_VIS; phase 1 = one_rk_step. */
+int    lfmo_stage_phase(void *h, int rk, int begin_step, int phase);
+void   lfmo_set_dt(void *h, double dt);
 
 /* Unit-level pieces (closed-form tests of the oracle itself). */
 double lfmo_minmod(int D, double cphi, double aphi, const double *grad, const double *d, double wl, double flux);

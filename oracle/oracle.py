@@ -23,7 +23,8 @@ class _Mesh(C.Structure):
                 ("face_off", C.c_void_p), ("face_pts", C.c_void_p), ("owner", C.c_void_p),
                 ("n_internal", C.c_int), ("neighbour", C.c_void_p), ("n_cells", C.c_int),
                 ("n_patches", C.c_int), ("patch_start", C.c_void_p), ("patch_size", C.c_void_p),
-                ("patch_role", C.c_void_p), ("patch_nbr", C.c_void_p), ("cell_submesh", C.c_void_p)]
+                ("patch_role", C.c_void_p), ("patch_nbr", C.c_void_p), ("cell_submesh", C.c_void_p),
+                ("face_key", C.c_void_p), ("rank", C.c_int)]
 
 
 class _Params(C.Structure):
@@ -73,6 +74,13 @@ def lib():
         L.lfmo_minmod.restype = C.c_double
         L.lfmo_minmod.argtypes = [C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_double, C.c_double]
         L.lfmo_p5.argtypes = [C.c_double, C.c_double, C.c_void_p]
+        L.lfmo_halo_nbrs.argtypes = [C.c_void_p, C.c_void_p]
+        L.lfmo_halo_export.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.lfmo_halo_import.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.lfmo_halo_pack.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.lfmo_halo_unpack.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.lfmo_stage_phase.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.lfmo_set_dt.argtypes = [C.c_void_p, C.c_double]
         L.lfmo_face_rhs.argtypes = [C.POINTER(_Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                     C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -148,6 +156,9 @@ class Oracle:
         m.patch_role = _ptr(arr(role, np.int32))
         m.patch_nbr = _ptr(arr(nbr, np.int32))
         m.cell_submesh = _ptr(arr(cell_submesh, np.int32)) if cell_submesh is not None else None
+        fk = getattr(mesh, "face_key", None)
+        m.face_key = _ptr(arr(fk, np.int32)) if fk is not None else None
+        m.rank = int(getattr(mesh, "rank", -1))
         pr = _Params()
         for k, v in params.items():
             if k == "U_inf":
@@ -166,6 +177,7 @@ class Oracle:
         self.h = L.lfmo_create(C.byref(m), C.byref(pr), _ptr(p), _ptr(T), _ptr(U), _ptr(a), err, 256)
         if not self.h:
             raise ValueError("oracle: " + err.value.decode())
+        self.params = dict(params)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -220,3 +232,82 @@ class Oracle:
     @property
     def n_ghost(self) -> int:
         return lib().lfmo_n_ghost(self.h)
+
+    # partitions --------------------------------------------------------------
+    def halo_neighbours(self):
+        n = lib().lfmo_halo_nbrs(self.h, None)
+        out = np.zeros(max(1, n), np.int32)
+        lib().lfmo_halo_nbrs(self.h, _ptr(out))
+        return out[:n].tolist()
+
+    def halo_export(self, rank: int):
+        n = lib().lfmo_halo_export(self.h, rank, None, None, None)
+        keys = np.zeros(n, np.int32)
+        cells = np.zeros(n, np.int32)
+        disp = np.zeros((n, 3))
+        lib().lfmo_halo_export(self.h, rank, _ptr(keys), _ptr(cells), _ptr(disp))
+        return keys, cells, disp
+
+    def halo_import(self, rank: int, msg):
+        keys, cells, disp = (np.ascontiguousarray(msg[0], np.int32), np.ascontiguousarray(msg[1], np.int32),
+                             np.ascontiguousarray(msg[2], np.float64))
+        rc = lib().lfmo_halo_import(self.h, rank, len(keys), _ptr(keys), _ptr(cells), _ptr(disp))
+        if rc != 0:
+            raise ValueError(f"oracle halo import from rank {rank} failed ({rc})")
+
+    def halo_pack(self, rank: int, kind: int) -> np.ndarray:
+        buf = np.zeros(64 + self.n_cells * 48)
+        n = lib().lfmo_halo_pack(self.h, rank, kind, _ptr(buf))
+        return buf[:n].copy()
+
+    def halo_unpack(self, rank: int, kind: int, buf: np.ndarray):
+        b = np.ascontiguousarray(buf, np.float64)
+        if lib().lfmo_halo_unpack(self.h, rank, kind, _ptr(b)) < 0:
+            raise ValueError("oracle halo unpack before import")
+
+    def stage_phase(self, rk: int, begin_step: bool, phase: int):
+        lib().lfmo_stage_phase(self.h, rk, int(begin_step), phase)
+
+    def set_dt(self, dt: float):
+        lib().lfmo_set_dt(self.h, dt)
+
+
+def connect(parts):
+    """The start-up exchange of neighbour centres between partitions."""
+    for q, o in enumerate(parts):
+        for r in o.halo_neighbours():
+            o.halo_import(r, parts[r].halo_export(q))
+
+
+def _exchange(parts, kind):
+    bufs = {(p, r): o.halo_pack(r, kind) for p, o in enumerate(parts) for r in o.halo_neighbours()}
+    for q, o in enumerate(parts):
+        for r in o.halo_neighbours():
+            o.halo_unpack(r, kind, bufs[(r, q)])
+
+
+def run_partitioned(parts, nsteps: int, with_grads: bool = False):
+    """Mesh::solve over ranks (src/mesh_solver.cpp:409-419, 496-717), ranks
+    interleaved phase by phase: solvars exchanged before every stage's
+    boundary conditions, viscous tensors after calc_VIS, dt reduced with
+    MIN over ranks when adjustTimeStep is on.  ``with_grads`` also sends the
+    M2AUSM minmod gradients with the viscous payload (not a reference
+    payload)."""
+    _exchange(parts, 1)
+    p0 = parts[0].params
+    nrk = int(p0["rk_order"])
+    for _ in range(nsteps):
+        for rk in range(nrk):
+            for o in parts:
+                o.stage_phase(rk, rk == 0, 0)
+            _exchange(parts, 6 if with_grads else 2)
+            for o in parts:
+                o.stage_phase(rk, rk == 0, 1)
+            _exchange(parts, 1)
+        if p0.get("adjust_dt"):
+            dt = min(o.compute_dt(p0["max_co"]) for o in parts)
+            for o in parts:
+                o.set_dt(dt)
+        else:
+            for o in parts:
+                o.compute_cfl(o.dt)
