@@ -36,12 +36,41 @@ def stage_bytes(nc, nf, D, w_s, w_g):
 
 
 def build_case(args):
+    """The workload and a one-line description of it."""
     from lfm_public_amd import cases
-    if args.workload == "cylinder2d":
-        return cases.cylinder_2d(n_theta=args.n_theta, n_quad=args.n_theta // 4, n_tri=3 * args.n_theta // 8)
-    if args.workload == "vortex2d":
-        return cases.isentropic_vortex(nx=args.n_theta // 4)
-    raise SystemExit(f"unknown workload {args.workload}")
+    w = args.workload
+    if w == "cylinder2d":
+        n = args.n_theta
+        c = cases.cylinder_2d(n_theta=n, n_quad=n // 4, n_tri=3 * n // 8, solver=args.solver)
+        desc = (f"2D mixed quad/tri annulus around a unit cylinder (10 % node jitter), M=0.3, 1e-4 p noise, "
+                f"solver {args.solver} ({'M2 + minmod AUSM dissipation' if args.solver == 2 else 'M2, minmodExists'})"
+                f", viscous, sponge Ls=5D, RK5")
+    elif w == "vortex2d":
+        c = cases.isentropic_vortex(nx=256)
+        desc = "2D 256x256 quad box, isentropic vortex (strength 5) at M=0.5, M1, inviscid, RK5, CFL 0.5"
+    elif w == "ogrid3d":
+        c = cases.cylinder_3d(nr=256, nth=256, nz=64, solver=1, minmod=1)
+        desc = ("3D cylinder O-grid 256x256x64 hexes, spanwise cyclic, M=0.2, mu from Re=3900, 1 % spanwise "
+                "perturbation, M2 + minmodExists, sponge Ls=13, RK5")
+    elif w == "box3d":
+        c = cases.acoustic_box(n=256)
+        desc = "3D 256^3 hex box, Gaussian pulse 1e-3 p_inf (half-width 8 cells) in M=0.1 flow, M1, inviscid, RK5"
+    else:
+        raise SystemExit(f"unknown workload {w}")
+    return c, desc
+
+
+def cpu_sample_case(args, case):
+    """The CPU baseline's sample: the workload itself up to ~1.5M cells, else
+    the same case family at a size one CPU solve finishes steps of in seconds."""
+    from lfm_public_amd import cases
+    if case.n_cells <= 1_500_000:
+        return case, "the full mesh"
+    if args.workload == "ogrid3d":
+        c = cases.cylinder_3d(nr=64, nth=64, nz=32, solver=1, minmod=1)
+    else:
+        c = cases.acoustic_box(n=64)
+    return c, f"the same case family at {c.n_cells} cells ({c.name})"
 
 
 def time_path(case, precision, steps, warmup):
@@ -83,22 +112,40 @@ def rel_errors(case, q, qref, D):
     return out
 
 
-def cpu_baseline(case, budget_s):
-    """The oracle (single-thread CPU restatement) on the same mesh, a bounded
-    number of steps."""
+def cpu_baseline(case, budget_s, threads, what="the full mesh"):
+    """The oracle (the C restatement of the reference's time step, one
+    thread per solve) on the same mesh, run as the reference runs on a CPU
+    node: one independent single-rank solve per core, all cores busy, a
+    bounded number of steps; the aggregate rate over the cores."""
+    import threading
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    o = O.Oracle(case.mesh, case.params, case.p, case.T, case.U, case.alpha)
-    t0 = time.perf_counter()
-    o.run(1)
-    t1 = time.perf_counter() - t0
+    insts = [None] * threads
+
+    def make(i):
+        insts[i] = O.Oracle(case.mesh, case.params, case.p, case.T, case.U, case.alpha)
+
+    def run(i, n):
+        insts[i].run(n)
+
+    def parallel(fn, *a):
+        ts = [threading.Thread(target=fn, args=(i,) + a) for i in range(threads)]
+        t0 = time.perf_counter()
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        return time.perf_counter() - t0
+
+    parallel(make)
+    t1 = parallel(run, 1)  # warm-up step (page faults, caches)
     n = max(1, min(10, int(budget_s / max(t1, 1e-3)) - 1))
-    t0 = time.perf_counter()
-    o.run(n)
-    dt = time.perf_counter() - t0
-    return {"value": case.n_cells * n / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (C restatement, 1 thread) on the full {case.n_cells}-cell mesh, {n} RK5 steps "
-                      f"after 1 warm-up step, FP64, {dt:.1f} s"}
+    wall = parallel(run, n)
+    return {"value": threads * case.n_cells * n / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C restatement of the reference time step, FP64) on {what} ({case.n_cells} cells): "
+                      f"{threads} independent single-thread solves on {threads} cores, {n} RK5 steps each after 1 "
+                      f"warm-up step, {wall:.1f} s; aggregate cell-updates/s (the reference itself needs MPI and "
+                      f"OpenFOAM, absent here)"}
 
 
 def main():
@@ -108,7 +155,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="cylinder2d")
     ap.add_argument("--n-theta", type=int, default=1024)
+    ap.add_argument("--solver", type=int, default=2, help="cylinder2d scheme: 2 (M2AUSM + minmod) or 1 (M2)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline cores (default: up to 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     args = ap.parse_args()
@@ -118,7 +167,7 @@ def main():
         sys.exit(2)
 
     from lfm_public_amd import capi
-    case = build_case(args)
+    case, desc = build_case(args)
     D = case.params["dim"]
     nc = case.n_cells
     print(f"[bench] workload {case.name}: {nc} cells", file=sys.stderr, flush=True)
@@ -161,20 +210,21 @@ def main():
     # rocprofv3 FETCH_SIZE/WRITE_SIZE passes committed under profiles/ for
     # this workload (calibrated on a plain copy), when they exist.
     nprof = max(1, min(args.steps, 10))
+    traffic_key = case.name + (f"_solver{args.solver}" if args.workload == "cylinder2d" and args.solver != 2 else "")
 
     def traffic_of(path):
-        f = os.path.join(ROOT, "profiles", f"{case.name}_traffic.json")
+        f = os.path.join(ROOT, "profiles", f"{traffic_key}_traffic.json")
         try:
             with open(f) as fh:
                 t = json.load(fh)
         except (OSError, ValueError):
             return None, None
-        if t.get("workload") != case.name:
+        if t.get("workload") != traffic_key:
             return None, None
         st = t.get("per_step", {}).get(path)
         if not st:
             return None, None
-        src = (f"profiles/{case.name}_traffic.json: rocprofv3 FETCH_SIZE x{t['fetch_factor']:.3f} + "
+        src = (f"profiles/{traffic_key}_traffic.json: rocprofv3 FETCH_SIZE x{t['fetch_factor']:.3f} + "
                f"WRITE_SIZE x{t['write_factor']:.3f} (factors from a 256 MiB device copy), summed over "
                + ", ".join(f"{n} x {k}" for k, n in st["kernels"].items()))
         return int(st["total"]), src
@@ -206,7 +256,18 @@ def main():
         value, ms_step, dtype = rate64, ms64 / args.steps, "f64"
         rl = roof(ms_step, prof64, 8, "fp64")
 
-    cpu = None if args.no_cpu else cpu_baseline(case, args.cpu_budget)
+    threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+    cpu = None
+    if not args.no_cpu:
+        cc, what = cpu_sample_case(args, case)
+        # one oracle instance holds ~0.7 kB (2D) / ~1.2 kB (3D) per cell
+        per = cc.n_cells * (700 if D == 2 else 1200)
+        try:
+            avail = int([l for l in open("/proc/meminfo") if l.startswith("MemAvailable")][0].split()[1]) * 1024
+            threads = max(1, min(threads, int(0.5 * avail // per)))
+        except (OSError, IndexError, ValueError):
+            pass
+        cpu = cpu_baseline(cc, args.cpu_budget, threads, what)
     out = {
         "metric": "cell-updates/s FP64 vs FP32-perturbation at 1e-5 rel err; HBM GB/s; p99",
         "value": value,
@@ -220,8 +281,7 @@ def main():
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": f"{case.name}: 2D mixed quad/tri annulus, {nc} cells, {nf} faces, M=0.3, "
-                               f"solver 2 (M2 + minmod AUSM dissipation), viscous, sponge Ls=5D, RK5",
+        "config": {"workload": f"{case.name}: {desc}; {nc} cells, {nf} faces",
                    "cells": nc, "faces": nf, "rk_stages": 5, "precision_paths": ["fp64"] + (["fp32p"] if have32 else [])},
         "fp64": {"cell_updates_per_s": rate64, "ms_per_step": ms64 / args.steps,
                  "roofline": roof(ms64 / args.steps, prof64, 8, "fp64")},

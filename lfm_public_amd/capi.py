@@ -31,7 +31,8 @@ EXPORTS = ("lfm_patch_role", "lfm_solver_create", "lfm_solver_destroy", "lfm_las
            "lfm_batch_set", "lfm_batch_run", "lfm_batch_synchronize", "lfm_solver_create_part",
            "lfm_halo_neighbours", "lfm_halo_export", "lfm_halo_import", "lfm_halo_sizes", "lfm_halo_pack",
            "lfm_halo_unpack", "lfm_part_group_create", "lfm_part_group_run", "lfm_part_group_synchronize",
-           "lfm_part_group_destroy")
+           "lfm_part_group_destroy", "lfm_case_open", "lfm_case_close", "lfm_case_mesh", "lfm_case_params",
+           "lfm_case_fields", "lfm_case_time", "lfm_case_dict_scalar", "lfm_case_write_time")
 HALO_SOLVARS, HALO_VISCOUS, HALO_PACKED = 1, 2, 3
 
 
@@ -202,6 +203,28 @@ class GpuSolver:
     @classmethod
     def from_case(cls, case, precision: int = PREC_FP64, **kw):
         return cls(case.mesh, case.params, case.p, case.T, case.U, case.alpha, precision=precision, **kw)
+
+    @classmethod
+    def from_descriptors(cls, m: PolyMeshDesc, part: Optional[PartitionDesc], prm: Params, fields) -> "GpuSolver":
+        """A solver from C-ABI descriptors (as lfm_case_mesh / lfm_case_params
+        fill them); ``fields`` has p, T, U (n x 3) and alpha arrays."""
+        L = load()
+        self = cls.__new__(cls)
+        self.h = None
+        self.dim = prm.dim
+        self.n_cells = m.n_cells
+        keep = [np.ascontiguousarray(fields.p, np.float64), np.ascontiguousarray(fields.T, np.float64),
+                np.ascontiguousarray(np.asarray(fields.U, np.float64).reshape(-1, 3)),
+                None if fields.alpha is None else np.ascontiguousarray(fields.alpha, np.float64)]
+        h = C.c_void_p()
+        if part is not None:
+            _check(L.lfm_solver_create_part(C.byref(m), C.byref(part), C.byref(prm), _ptr(keep[0]), _ptr(keep[1]),
+                                            _ptr(keep[2]), _ptr(keep[3]), C.byref(h)))
+        else:
+            _check(L.lfm_solver_create(C.byref(m), C.byref(prm), _ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]),
+                                       _ptr(keep[3]), C.byref(h)))
+        self.h = h.value
+        return self
 
     def close(self):
         if getattr(self, "h", None):

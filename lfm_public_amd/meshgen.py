@@ -70,6 +70,9 @@ class PolyMesh:
     # global face index shared by both sides of a processor face (-1 else)
     rank: int = -1
     face_key: Optional[np.ndarray] = None
+    # decomposePar's faceProcAddressing: +-(global face + 1), negative where
+    # the face is reversed (None for an undecomposed mesh)
+    face_proc: Optional[np.ndarray] = None
 
     @property
     def n_faces(self) -> int:
@@ -528,9 +531,11 @@ def decompose(mesh: PolyMesh, cell_rank: np.ndarray):
             if is_proc:
                 key[off - fint.shape[0]:off - fint.shape[0] + fs.shape[0]] = fs
             off += fs.shape[0]
+        fproc = (faces.astype(np.int64) + 1) * np.where(flip_all, -1, 1)
         lm = PolyMesh(np.ascontiguousarray(mesh.points[used]), np.asarray(offs, np.int32),
                       np.concatenate(flat).astype(np.int32), lown.astype(np.int32), lnbr.astype(np.int32),
-                      int(cells.shape[0]), patches, mesh.dim, rank=r, face_key=key)
+                      int(cells.shape[0]), patches, mesh.dim, rank=r, face_key=key,
+                      face_proc=fproc.astype(np.int32))
         out.append((lm, cells))
     return out
 
