@@ -109,6 +109,11 @@ def rel_errors(case, q, qref, D):
         uinf = case.params["U_inf"][k]
         out[f"U{k}"] = {"rel_l2": float(np.linalg.norm(U[k] - Ur[k]) / max(np.linalg.norm(Ur[k]), 1e-300)),
                         "pert_l2": float(np.linalg.norm(U[k] - Ur[k]) / max(np.linalg.norm(Ur[k] - uinf), 1e-300))}
+    # the velocity vector as one field (a component that is itself a small
+    # perturbation, like a spanwise velocity, is ill-conditioned on its own)
+    uinf = np.asarray(case.params["U_inf"][:D], np.float64)[:, None]
+    out["U"] = {"rel_l2": float(np.linalg.norm(U - Ur) / np.linalg.norm(Ur)),
+                "pert_l2": float(np.linalg.norm(U - Ur) / max(np.linalg.norm(Ur - uinf), 1e-300))}
     return out
 
 

@@ -31,6 +31,7 @@ struct Batch {
     BatchMap map_cells{}, map_faces{};
     int nb_cells = 0, nb_faces = 0;
     int D = 2, solver = 0, minmod = 0, nstages = 0;
+    bool visc = true;
     double A[5]{}, B[5]{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -44,19 +45,24 @@ struct Batch {
     }
 };
 
-template <int D>
+template <int D, bool V>
 static void enqueue_step(Batch *b) {
     const dim3 blk(256);
     for (int rk = 0; rk < b->nstages; rk++) {
-        // ghosts are refreshed by the update kernel (members join with fresh ghosts)
-        if (b->solver == 2 && b->minmod)
-            hipLaunchKernelGGL((grad_vis32_b<D, 2>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args, b->map_cells);
-        else
-            hipLaunchKernelGGL((grad_vis32_b<D, 0>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args, b->map_cells);
+        // ghosts are refreshed by the update kernel (members join with fresh ghosts);
+        // without viscosity solvers 0/1 consume no gradient or tensor
+        if (V || b->solver == 2) {
+            if (b->solver == 2 && b->minmod)
+                hipLaunchKernelGGL((grad_vis32_b<D, 2, V>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
+                                   b->map_cells);
+            else
+                hipLaunchKernelGGL((grad_vis32_b<D, 0, V>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
+                                   b->map_cells);
+        }
         switch (b->solver) {
-        case 0: hipLaunchKernelGGL((face_flux32_b<D, 0>), dim3(b->nb_faces), blk, 0, b->stream, b->d_args, b->map_faces); break;
-        case 1: hipLaunchKernelGGL((face_flux32_b<D, 1>), dim3(b->nb_faces), blk, 0, b->stream, b->d_args, b->map_faces); break;
-        default: hipLaunchKernelGGL((face_flux32_b<D, 2>), dim3(b->nb_faces), blk, 0, b->stream, b->d_args, b->map_faces); break;
+        case 0: hipLaunchKernelGGL((face_flux32_b<D, 0, V>), dim3(b->nb_faces), blk, 0, b->stream, b->d_args, b->map_faces); break;
+        case 1: hipLaunchKernelGGL((face_flux32_b<D, 1, V>), dim3(b->nb_faces), blk, 0, b->stream, b->d_args, b->map_faces); break;
+        default: hipLaunchKernelGGL((face_flux32_b<D, 2, V>), dim3(b->nb_faces), blk, 0, b->stream, b->d_args, b->map_faces); break;
         }
         hipLaunchKernelGGL(cell_update32_b<D>, dim3(b->nb_cells), blk, 0, b->stream, b->d_args, b->map_cells,
                            (float)b->A[rk], (float)b->B[rk], rk == 0 ? 1 : 0);
@@ -98,13 +104,15 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
             b->solver = s->prm.solver;
             b->minmod = s->prm.minmod;
             b->nstages = s->nstages;
+            b->visc = s->visc;
             for (int k = 0; k < 5; k++) {
                 b->A[k] = s->A[k];
                 b->B[k] = s->B[k];
             }
         } else if (s->L.D != b->D || s->prm.solver != b->solver || s->prm.minmod != b->minmod ||
-                   s->nstages != b->nstages) {
-            return fail(LFM_EINVAL, "batched solvers must share dimension, solver, minmod and rkOrder");
+                   s->nstages != b->nstages || s->visc != b->visc) {
+            return fail(LFM_EINVAL, "batched solvers must share dimension, solver, minmod, rkOrder and whether "
+                                    "they are viscous");
         }
         for (Solver *o : b->members)
             if (o == s) return fail(LFM_EINVAL, "solver listed twice in a batch");
@@ -152,9 +160,9 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
     // capture one RK step; on failure leave no capture or sticky error behind
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     if (b->D == 2)
-        enqueue_step<2>(b);
+        b->visc ? enqueue_step<2, true>(b) : enqueue_step<2, false>(b);
     else
-        enqueue_step<3>(b);
+        b->visc ? enqueue_step<3, true>(b) : enqueue_step<3, false>(b);
     hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {

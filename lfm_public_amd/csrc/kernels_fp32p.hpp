@@ -191,7 +191,7 @@ __device__ __forceinline__ void bc_fill32_body(const KArgs32 &a, const int g) {
 // ---------------------------------------------------------------------------
 // Green-Gauss of deviations + viscous tensors, per cell
 // ---------------------------------------------------------------------------
-template <int D, int GRAD>
+template <int D, int GRAD, bool VISC>
 __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
     if (c >= a.nc) return;
     const float vinv = a.vinv[c];
@@ -226,12 +226,12 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
         float fU[D];
 #pragma unroll
         for (int k = 0; k < D; k++) fU[k] = LERP32(w, sp.up[k], sn.up[k]);
-        const float fT = LERP32(w, sp.Rp, sn.Rp) * a.R_inv;
+        const float fT = VISC ? LERP32(w, sp.Rp, sn.Rp) * a.R_inv : 0.f;
 #pragma unroll
         for (int i = 0; i < D; i++) {
 #pragma unroll
             for (int j = 0; j < D; j++) dudx[i][j] += fU[i] * sv[j];
-            dTdx[i] += fT * sv[i];
+            if (VISC) dTdx[i] += fT * sv[i];
         }
         if (GRAD == 2) {
             const float rf = LERP32(w, sp.rp, sn.rp);
@@ -266,15 +266,15 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
     const size_t nt = a.ntot;
 #pragma unroll
     for (int i = 0; i < D; i++) {
-        a.sigmaU[i * nt + c] = dot32<D>(sc.u, tau[i]);
+        if (VISC) a.sigmaU[i * nt + c] = dot32<D>(sc.u, tau[i]);
 #pragma unroll
         for (int j = 0; j < D; j++) {
             float t = mu * dudx[j][i];
             if (i == j) t += diag;
-            a.tauMC[(i * D + j) * nt + c] = t;
+            if (VISC) a.tauMC[(i * D + j) * nt + c] = t;
             a.dudx[(i * D + j) * nt + c] = dudx[i][j];
         }
-        a.dTdx[i * nt + c] = dTdx[i];
+        if (VISC) a.dTdx[i * nt + c] = dTdx[i];
     }
     if (GRAD == 2) {
 #pragma unroll
@@ -290,7 +290,7 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
 // ---------------------------------------------------------------------------
 // face fluxes minus the freestream flux, one thread per face
 // ---------------------------------------------------------------------------
-template <int D, int SOLVER>
+template <int D, int SOLVER, bool VISC>
 __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) {
     if (f >= a.nf) return;
     const int P = a.fP[f], N = a.fN[f];
@@ -456,6 +456,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
     }
 
     // viscous terms (cfd_v0.cpp:2682-2790) on velocity / temperature deviations
+    if (VISC) {
     const float mu = a.mu;
     float dmag2 = 0.f;
 #pragma unroll
@@ -535,6 +536,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
     for (int i = 0; i < D; i++)
         rhs[i + 1] += dot32<D>(tmc[i], S) + mu * (dm * (sn.up[i] - sp.up[i]) * dmi + dot32<D>(K, du[i]));
     rhs[D + 1] += dot32<D>(sU, S) + kcond * (dm * (aT - cT) * dmi + dot32<D>(K, dT));
+    }
 #pragma unroll
     for (int k = 0; k < D + 2; k++) a.frhs[(size_t)k * a.nf + f] = rhs[k];
 }
@@ -729,13 +731,13 @@ template <int D>
 __global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
     bc_fill32_body<D>(a, LFM_TID(blockIdx.x));
 }
-template <int D, int GRAD>
+template <int D, int GRAD, bool VISC>
 __global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
-    grad_vis32_body<D, GRAD>(a, LFM_TID(blockIdx.x));
+    grad_vis32_body<D, GRAD, VISC>(a, LFM_TID(blockIdx.x));
 }
-template <int D, int SOLVER>
+template <int D, int SOLVER, bool VISC>
 __global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
-    face_flux32_body<D, SOLVER>(a, LFM_TID(blockIdx.x));
+    face_flux32_body<D, SOLVER, VISC>(a, LFM_TID(blockIdx.x));
 }
 template <int D, bool RES>
 __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B, int zero_dq) {
@@ -763,17 +765,17 @@ __device__ __forceinline__ void batch_slot(const BatchMap &m, int &r, int &lblk)
     lblk = b - m.first[r];
 }
 
-template <int D, int GRAD>
+template <int D, int GRAD, bool VISC>
 __global__ void __launch_bounds__(256) grad_vis32_b(const KArgs32 *args, BatchMap m) {
     int r, lb;
     batch_slot(m, r, lb);
-    grad_vis32_body<D, GRAD>(args[r], lb * 256 + threadIdx.x);
+    grad_vis32_body<D, GRAD, VISC>(args[r], lb * 256 + threadIdx.x);
 }
-template <int D, int SOLVER>
+template <int D, int SOLVER, bool VISC>
 __global__ void __launch_bounds__(256) face_flux32_b(const KArgs32 *args, BatchMap m) {
     int r, lb;
     batch_slot(m, r, lb);
-    face_flux32_body<D, SOLVER>(args[r], lb * 256 + threadIdx.x);
+    face_flux32_body<D, SOLVER, VISC>(args[r], lb * 256 + threadIdx.x);
 }
 template <int D>
 __global__ void __launch_bounds__(256) cell_update32_b(const KArgs32 *args, BatchMap m, float A, float B,
@@ -812,7 +814,7 @@ static __global__ void __launch_bounds__(256) cfl_final32_b(const KArgs32 *args)
 template <int D>
 void launch_bc32(Solver *s);
 template <int D>
-void launch_gradvis32(Solver *s);
+void launch_gradvis32(Solver *s, bool full);
 template <int D>
 void launch_flux32(Solver *s);
 template <int D>

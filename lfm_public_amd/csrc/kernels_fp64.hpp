@@ -195,8 +195,11 @@ __global__ void __launch_bounds__(256) bc_fill(KArgs a) {
 // stored values equal the refresh of the completed sums done here.
 // GRAD: 0 none, 1 calc_gradients (results unused by solvers 0/1: skipped),
 // 2 calc_gradients_M2AUSM (rho, p, U gradients consumed by the M2AUSM flux).
+// VISC false (mu = 0): dTdx, tau, sigmaU and tauMC only ever enter the flux
+// multiplied by mu, so they are not formed; dudx is kept for the M2AUSM
+// sensor.  (For solvers 0/1 with mu = 0 the kernel is not launched.)
 // ---------------------------------------------------------------------------
-template <int D, int GRAD>
+template <int D, int GRAD, bool VISC>
 __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
     const int c = LFM_TID(blockIdx.x);
     if (c >= a.nc) return;
@@ -266,12 +269,12 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
         double fU[D];
 #pragma unroll
         for (int k = 0; k < D; k++) fU[k] = INTERP_LINEAR(w, qP[k + 1] * cri, qN[k + 1] * ari);
-        const double fT = INTERP_LINEAR(w, cR, aR) * a.R_inv;
+        const double fT = VISC ? INTERP_LINEAR(w, cR, aR) * a.R_inv : 0.0;
 #pragma unroll
         for (int i = 0; i < D; i++) {
 #pragma unroll
             for (int j = 0; j < D; j++) dudx[i][j] += fU[i] * sv[j];
-            dTdx[i] += fT * sv[i];
+            if (VISC) dTdx[i] += fT * sv[i];
         }
     }
     // refresh of tau / sigmaU / tauMC from the completed dudx (cfd_v0.cpp:1813-1834)
@@ -299,15 +302,15 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
     const size_t nt = a.ntot;
 #pragma unroll
     for (int i = 0; i < D; i++) {
-        a.sigmaU[i * nt + c] = dotD<D>(U, tau[i]);
+        if (VISC) a.sigmaU[i * nt + c] = dotD<D>(U, tau[i]);
 #pragma unroll
         for (int j = 0; j < D; j++) {
             double t = mu * dudx[j][i];
             if (i == j) t += diag;
-            a.tauMC[(i * D + j) * nt + c] = t;
+            if (VISC) a.tauMC[(i * D + j) * nt + c] = t;
             a.dudx[(i * D + j) * nt + c] = dudx[i][j];
         }
-        a.dTdx[i * nt + c] = dTdx[i];
+        if (VISC) a.dTdx[i * nt + c] = dTdx[i];
     }
     if (GRAD == 2) {
 #pragma unroll
@@ -323,8 +326,10 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
 // ---------------------------------------------------------------------------
 // Face fluxes: convective (M1 2589-2675, M2 1968-2023, M2AUSM 2256-2366) plus
 // the viscous terms shared by all three (2682-2790).  One thread per face.
+// VISC false (mu = 0): every viscous term is mu or k = cp mu / Pr times a
+// finite value, i.e. a zero the reference adds; it is not formed.
 // ---------------------------------------------------------------------------
-template <int D, int SOLVER>
+template <int D, int SOLVER, bool VISC>
 __global__ void __launch_bounds__(256) face_flux(KArgs a) {
     const int f = LFM_TID(blockIdx.x);
     if (f >= a.nf) return;
@@ -505,6 +510,7 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
     }
 
     // viscous terms (cfd_v0.cpp:2682-2790)
+    if (VISC) {
     const double mu = a.mu;
     double dmag = 0.0;
 #pragma unroll
@@ -588,6 +594,7 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
     }
     const double lapT = kcond * (dm * (aT - cT) * dmi + dotD<D>(K, dT));
     rhs[D + 1] += dotD<D>(sU, S) + lapT;
+    }
 #pragma unroll
     for (int k = 0; k < D + 2; k++) a.frhs[(size_t)k * a.nf + f] = rhs[k];
 }

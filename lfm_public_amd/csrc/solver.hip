@@ -159,23 +159,41 @@ static void launch_bc64(Solver *s) {
     if (s->L.ng) hipLaunchKernelGGL(bc_fill<D>, dim3(nblocks(s->L.ng)), dim3(kBlock), 0, s->stream, s->k64);
 }
 
-template <int D>
-static void launch_gradvis64(Solver *s) {
+template <int D, bool V>
+static void launch_gradvis64_v(Solver *s) {
     const int nb = nblocks(s->L.nc);
     if (s->prm.solver == 2 && s->prm.minmod)
-        hipLaunchKernelGGL((grad_vis<D, 2>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+        hipLaunchKernelGGL((grad_vis<D, 2, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
     else
-        hipLaunchKernelGGL((grad_vis<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+        hipLaunchKernelGGL((grad_vis<D, 0, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+}
+
+// full: every tensor calc_VIS forms (the per-stage entry point); otherwise
+// only what the step consumes (nothing for solvers 0/1 when mu = 0)
+template <int D>
+static void launch_gradvis64(Solver *s, bool full) {
+    if (full || s->visc)
+        launch_gradvis64_v<D, true>(s);
+    else if (s->prm.solver == 2)
+        launch_gradvis64_v<D, false>(s);
+}
+
+template <int D, bool V>
+static void launch_flux64_v(Solver *s) {
+    const int nb = nblocks(s->L.nf);
+    switch (s->prm.solver) {
+    case 0: hipLaunchKernelGGL((face_flux<D, 0, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
+    case 1: hipLaunchKernelGGL((face_flux<D, 1, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
+    default: hipLaunchKernelGGL((face_flux<D, 2, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
+    }
 }
 
 template <int D>
 static void launch_flux64(Solver *s) {
-    const int nb = nblocks(s->L.nf);
-    switch (s->prm.solver) {
-    case 0: hipLaunchKernelGGL((face_flux<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
-    case 1: hipLaunchKernelGGL((face_flux<D, 1>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
-    default: hipLaunchKernelGGL((face_flux<D, 2>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
-    }
+    if (s->visc)
+        launch_flux64_v<D, true>(s);
+    else
+        launch_flux64_v<D, false>(s);
 }
 
 template <int D>
@@ -233,11 +251,11 @@ void Solver::bc() {
     }
 }
 
-void Solver::gradvis() {
+void Solver::gradvis(bool full) {
     if (fp32) {
-        LFM_DISPATCH_D(this, launch_gradvis32<2>(this), launch_gradvis32<3>(this));
+        LFM_DISPATCH_D(this, launch_gradvis32<2>(this, full), launch_gradvis32<3>(this, full));
     } else {
-        LFM_DISPATCH_D(this, launch_gradvis64<2>(this), launch_gradvis64<3>(this));
+        LFM_DISPATCH_D(this, launch_gradvis64<2>(this, full), launch_gradvis64<3>(this, full));
     }
 }
 
@@ -288,7 +306,7 @@ void Solver::stage_a(int rk) {
         ghosts_stale = false;
     }
     prof_mark(1, true);
-    gradvis();
+    gradvis(false);
     prof_mark(1, false);
 }
 
@@ -370,6 +388,7 @@ int lfm_solver_create_part(const lfm_polymesh *mesh, const lfm_partition *part, 
     s->nstages = lfm::rk_tables(prm->rk_order, s->A, s->B);
     s->dt = prm->dt;
     s->fp32 = prm->precision == LFM_PREC_FP32P;
+    s->visc = prm->mu != 0.0;
     std::lock_guard<std::mutex> lock(lfm::device_setup_mutex());
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&s->ev0));
@@ -459,7 +478,7 @@ int lfm_calc_gradients(lfm_solver *h) {
 int lfm_calc_vis(lfm_solver *h) {
     if (!h) return fail(LFM_EINVAL, "null handle");
     if (!h->impl->halo_ready()) return fail(LFM_ESTATE, "halo set-up not imported for every neighbour");
-    h->impl->gradvis();
+    h->impl->gradvis(true);
     HIP_TRY(hipGetLastError());
     return LFM_OK;
 }

@@ -153,23 +153,39 @@ void launch_bc32(Solver *s) {
     if (s->L.ng) hipLaunchKernelGGL(bc_fill32<D>, dim3(nblocks(s->L.ng)), dim3(kBlock), 0, s->stream, *s->k32);
 }
 
-template <int D>
-void launch_gradvis32(Solver *s) {
+template <int D, bool V>
+static void launch_gradvis32_v(Solver *s) {
     const int nb = nblocks(s->L.nc);
     if (s->prm.solver == 2 && s->prm.minmod)
-        hipLaunchKernelGGL((grad_vis32<D, 2>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+        hipLaunchKernelGGL((grad_vis32<D, 2, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
     else
-        hipLaunchKernelGGL((grad_vis32<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+        hipLaunchKernelGGL((grad_vis32<D, 0, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+}
+
+template <int D>
+void launch_gradvis32(Solver *s, bool full) {
+    if (full || s->visc)
+        launch_gradvis32_v<D, true>(s);
+    else if (s->prm.solver == 2)
+        launch_gradvis32_v<D, false>(s);
+}
+
+template <int D, bool V>
+static void launch_flux32_v(Solver *s) {
+    const int nb = nblocks(s->L.nf);
+    switch (s->prm.solver) {
+    case 0: hipLaunchKernelGGL((face_flux32<D, 0, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
+    case 1: hipLaunchKernelGGL((face_flux32<D, 1, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
+    default: hipLaunchKernelGGL((face_flux32<D, 2, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
+    }
 }
 
 template <int D>
 void launch_flux32(Solver *s) {
-    const int nb = nblocks(s->L.nf);
-    switch (s->prm.solver) {
-    case 0: hipLaunchKernelGGL((face_flux32<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
-    case 1: hipLaunchKernelGGL((face_flux32<D, 1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
-    default: hipLaunchKernelGGL((face_flux32<D, 2>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
-    }
+    if (s->visc)
+        launch_flux32_v<D, true>(s);
+    else
+        launch_flux32_v<D, false>(s);
 }
 
 template <int D>
@@ -265,8 +281,8 @@ int nonfinite32(Solver *s, unsigned long long *d) {
 
 template void launch_bc32<2>(Solver *);
 template void launch_bc32<3>(Solver *);
-template void launch_gradvis32<2>(Solver *);
-template void launch_gradvis32<3>(Solver *);
+template void launch_gradvis32<2>(Solver *, bool);
+template void launch_gradvis32<3>(Solver *, bool);
 template void launch_flux32<2>(Solver *);
 template void launch_flux32<3>(Solver *);
 template void launch_update32<2>(Solver *, double, double, bool, bool, bool);

@@ -33,6 +33,7 @@ struct Solver {
     int nstages = 0;
     double dt = 0.0;
     bool fp32 = false;
+    bool visc = true;  // mu != 0: the step forms the viscous terms
 
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -91,7 +92,7 @@ struct Solver {
 
     // stage pieces on the handle's stream
     void bc();
-    void gradvis();
+    void gradvis(bool full);
     void flux();
     void update(double A, double B, bool zero_dq, bool res, bool accumulate_res);
     void cfl_dt(int mode, double param, double *out, double *dt_dst);
