@@ -57,6 +57,10 @@ def build_case(args):
         desc = "3D 256^3 hex box, Gaussian pulse 1e-3 p_inf (half-width 8 cells) in M=0.1 flow, M1, inviscid, RK5"
     else:
         raise SystemExit(f"unknown workload {w}")
+    if args.renumber != "none":
+        from lfm_public_amd import renumber
+        c = renumber.renumber_case(c, args.renumber)
+        desc += f"; cells renumbered ({args.renumber})"
     return c, desc
 
 
@@ -161,6 +165,8 @@ def main():
     ap.add_argument("--workload", default="cylinder2d")
     ap.add_argument("--n-theta", type=int, default=1024)
     ap.add_argument("--solver", type=int, default=2, help="cylinder2d scheme: 2 (M2AUSM + minmod) or 1 (M2)")
+    ap.add_argument("--renumber", default="none", choices=("none", "morton", "rcm"),
+                    help="renumber the cells before the run (renumberMesh-style preprocessing)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline cores (default: up to 16)")
     ap.add_argument("--no-cpu", action="store_true")

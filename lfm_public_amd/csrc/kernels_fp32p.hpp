@@ -797,6 +797,8 @@ static __global__ void __launch_bounds__(256) cfl_final32_b(const KArgs32 *args)
     __shared__ double sh[256];
     const KArgs32 &a = args[blockIdx.x];
     double v = 0.0;
+    // min / max are exact in any order: keep several loads in flight
+#pragma unroll 8
     for (int i = threadIdx.x; i < a.nblk_cells; i += blockDim.x) {
         const double o = a.partial[i];
         if (v < o) v = o;
