@@ -32,6 +32,7 @@ struct KArgs32 {
     const int32_t *g_off, *g_ent, *s_off, *s_ent;
     const int32_t *ghost_cell, *ghost_role;
     const float *vinv, *sigma;
+    const float *smag;  // [nc] Smagorinsky constants, nullptr when laminar
     float *q, *dq, *res_cell;
     float *dudx, *dTdx, *tauMC, *sigmaU;
     float *rho_grad, *p_grad, *U_grad;
@@ -264,14 +265,39 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
     for (int p = 0; p < D; p++) diag -= dudx[p][p];
     diag *= mu * 2.0f / 3.0f;
     const size_t nt = a.ntot;
+    float tmc[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+#pragma unroll
+        for (int j = 0; j < D; j++) tmc[i][j] = mu * dudx[j][i];
+        tmc[i][i] += diag;
+    }
+    if (VISC && a.smag) {  // Smagorinsky stress (cfd_v0.cpp:1668-1684)
+        const float smag = a.smag[c];
+        float strain = 0.f, divu = 0.f;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                const float sij = 0.5f * (dudx[i][j] + dudx[j][i]);
+                strain += sij * sij;
+            }
+            divu += dudx[i][i];
+        }
+        strain = sqrt32(2.f * strain);
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) tmc[i][j] -= smag * strain * (0.5f * (dudx[i][j] + dudx[j][i]));
+            tmc[i][i] += smag * strain * divu * (1.f / 3.f);
+        }
+    }
 #pragma unroll
     for (int i = 0; i < D; i++) {
         if (VISC) a.sigmaU[i * nt + c] = dot32<D>(sc.u, tau[i]);
 #pragma unroll
         for (int j = 0; j < D; j++) {
-            float t = mu * dudx[j][i];
-            if (i == j) t += diag;
-            if (VISC) a.tauMC[(i * D + j) * nt + c] = t;
+            if (VISC) a.tauMC[(i * D + j) * nt + c] = tmc[i][j];
             a.dudx[(i * D + j) * nt + c] = dudx[i][j];
         }
         if (VISC) a.dTdx[i * nt + c] = dTdx[i];

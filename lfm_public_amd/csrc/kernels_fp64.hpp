@@ -29,6 +29,7 @@ struct KArgs {
     const int32_t *s_off, *s_ent;   // slot lists
     const int32_t *ghost_cell, *ghost_role;
     const double *vinv, *sigma;
+    const double *smag;             // [nc] Smagorinsky constants, nullptr when laminar
     double *q;                      // [(D+2)][ntot]
     double *dq;                     // [(D+2)][nc]
     double *res_cell;               // [(D+2)][nc]
@@ -300,14 +301,44 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
     for (int p = 0; p < D; p++) diag -= dudx[p][p];
     diag *= mu * 2.0 / 3.0;
     const size_t nt = a.ntot;
+    double tmc[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+#pragma unroll
+        for (int j = 0; j < D; j++) tmc[i][j] = mu * dudx[j][i];
+        tmc[i][i] += diag;
+    }
+    if (VISC && a.smag) {
+        // calc_VIS_Smagorinsky (cfd_v0.cpp:1668-1684): minus the eddy stress
+        // smag |S| S_ij, plus the compressible correction smag |S| div u / 3
+        const double smag = a.smag[c];
+        double strain = 0.0, divu = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                const double sij = 0.5 * (dudx[i][j] + dudx[j][i]);
+                strain += sij * sij;
+            }
+            divu += dudx[i][i];
+        }
+        strain = sqrt(2.0 * strain);
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                const double sij = 0.5 * (dudx[i][j] + dudx[j][i]);
+                tmc[i][j] -= smag * strain * sij;
+            }
+            tmc[i][i] += smag * strain * divu / 3.0;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < D; i++) {
         if (VISC) a.sigmaU[i * nt + c] = dotD<D>(U, tau[i]);
 #pragma unroll
         for (int j = 0; j < D; j++) {
-            double t = mu * dudx[j][i];
-            if (i == j) t += diag;
-            if (VISC) a.tauMC[(i * D + j) * nt + c] = t;
+            if (VISC) a.tauMC[(i * D + j) * nt + c] = tmc[i][j];
             a.dudx[(i * D + j) * nt + c] = dudx[i][j];
         }
         if (VISC) a.dTdx[i * nt + c] = dTdx[i];

@@ -507,6 +507,24 @@ std::string build_layout(const lfm_polymesh &m, const lfm_partition *part, const
         L.centre[3 * ci + 1] = cc[c].y;
         L.centre[3 * ci + 2] = cc[c].z;
     }
+    if (!prm.laminar) {
+        // calc_VIS_Smagorinsky (cfd_v0.cpp:1594-1733) refreshes tauMC at every
+        // touch with the constant of the cell whose faces are processed; the
+        // last touch of a cell is in its own loop when it has a face to
+        // process, else the last face that reached it from an earlier cell
+        auto cst = [&](int ci) { return -2.0 * std::pow(0.16 * std::pow(1.0 / L.vinv[ci], 1. / 3.), 2); };
+        L.smag.resize(nc);
+        for (int ci = 0; ci < nc; ci++) {
+            int last_p = -1;
+            bool own = false;
+            for (int e = L.g_off[ci]; e < L.g_off[ci + 1]; e++) {
+                const int ent = L.g_ent[e];
+                if (ent >= 0) own = true;
+                else last_p = L.fP[~ent];
+            }
+            L.smag[ci] = own || last_p < 0 ? cst(ci) : cst(last_p);
+        }
+    }
     if (prm.Ls > 0) {
         if (!alpha) return "Ls > 0 needs the alpha field";
         const double sig0 = (3.0 * (1.0 - prm.mach * prm.mach) / (prm.k_sponge * prm.Ls));

@@ -47,6 +47,10 @@ struct SolverLayout {
     std::vector<int32_t> fP, fN;        // [nf]
     std::vector<double> S, d, w;        // S,d: [D][nf] SoA; w: [nf]
     std::vector<double> vinv, sigma;    // [nc]
+    // Smagorinsky constant -2 (0.16 V^(1/3))^2 each cell's stored tauMC ends
+    // with (empty when laminar): its own, or for a cell with no face of its
+    // own to process, that of the cell whose face touched it last
+    std::vector<double> smag;
     std::vector<double> centre, volume; // [nc*3], [nc] (combined order)
     // gather lists: per cell, faces touching it in processing order;
     // entry f >= 0: the cell is P of face f; entry ~f: the cell is N.

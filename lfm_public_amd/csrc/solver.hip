@@ -105,6 +105,7 @@ static int create_fp64(Solver *s) {
     rc |= dev_upload(s, (int32_t **)&a.ghost_role, L.ghost_role.data(), L.ng);
     rc |= dev_upload(s, (double **)&a.vinv, L.vinv.data(), nc);
     rc |= dev_upload(s, (double **)&a.sigma, L.sigma.data(), nc);
+    if (!L.smag.empty()) rc |= dev_upload(s, (double **)&a.smag, L.smag.data(), nc);
     rc |= dev_zeros(s, &a.q, (size_t)(D + 2) * ntot);
     rc |= dev_zeros(s, &a.dq, (size_t)(D + 2) * nc);
     rc |= dev_zeros(s, &a.res_cell, (size_t)(D + 2) * nc);
@@ -378,8 +379,6 @@ int lfm_solver_create_part(const lfm_polymesh *mesh, const lfm_partition *part, 
     if (prm->rk_order < 0 || prm->rk_order > 5) return fail(LFM_EINVAL, "rkOrder must be 0..5");
     if (prm->precision != LFM_PREC_FP64 && prm->precision != LFM_PREC_FP32P)
         return fail(LFM_EINVAL, "precision must be 64 or 32");
-    if (!prm->laminar)
-        return fail(LFM_EINVAL, "only laminar simulationType is supported (calc_VIS_Smagorinsky not built)");
     auto s = std::make_unique<Solver>();
     s->prm = *prm;
     std::string err = lfm::build_layout(*mesh, part, *prm, p, T, U3, alpha, s->L);
@@ -388,7 +387,8 @@ int lfm_solver_create_part(const lfm_polymesh *mesh, const lfm_partition *part, 
     s->nstages = lfm::rk_tables(prm->rk_order, s->A, s->B);
     s->dt = prm->dt;
     s->fp32 = prm->precision == LFM_PREC_FP32P;
-    s->visc = prm->mu != 0.0;
+    // the Smagorinsky stress does not scale with mu
+    s->visc = prm->mu != 0.0 || !prm->laminar;
     std::lock_guard<std::mutex> lock(lfm::device_setup_mutex());
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&s->ev0));

@@ -98,6 +98,10 @@ int create_fp32p(Solver *s) {
         auto vinv = to_f32(L.vinv), sig = to_f32(L.sigma);
         rc |= up(s, (float **)&a.vinv, vinv.data(), nc);
         rc |= up(s, (float **)&a.sigma, sig.data(), nc);
+        if (!L.smag.empty()) {
+            auto sm = to_f32(L.smag);
+            rc |= up(s, (float **)&a.smag, sm.data(), nc);
+        }
     }
     rc |= up(s, (int32_t **)&a.g_off, L.g_off.data(), nc + 1);
     rc |= up(s, (int32_t **)&a.g_ent, L.g_ent.data(), L.g_ent.size());

@@ -70,7 +70,7 @@ typedef struct {
 } cellgeo;
 
 typedef struct {
-    int D, solver, nrk, minmod, adjust, have_res;
+    int D, solver, nrk, minmod, adjust, have_res, laminar;
     double max_co, dt, cfl;
     int nc, ng, nsm0;
     int *orig;            /* combined index -> original cell id        */
@@ -298,6 +298,7 @@ void *lfmo_create(const lfmo_mesh *m, const lfmo_params *prm,
     o->minmod = prm->minmod;
     o->adjust = prm->adjust_dt;
     o->have_res = prm->have_res;
+    o->laminar = prm->laminar;
     o->max_co = prm->max_co;
     o->dt = prm->dt;
     o->nc = nc;
@@ -904,6 +905,66 @@ static void calc_VIS(oracle *o)
     }
 }
 
+/* calc_VIS_Smagorinsky (cfd_v0.cpp:1574-1737): calc_VIS's accumulation, and
+ * at every touch of a cell the laminar refresh plus the constant-coefficient
+ * Smagorinsky stress with the constant -2 (0.16 V^(1/3))^2 of the cell whose
+ * faces are being processed (also when refreshing the neighbour). */
+static void vis_refresh_smag(const oracle *o, solvars *v, const double *U, double smag)
+{
+    const int D = o->D;
+    vis_refresh(o, v, U);
+    double strain = 0.0, divu = 0.0, sij;
+    for (int i = 0; i < D; i++) {
+        for (int j = 0; j < D; j++) {
+            sij = 0.5 * (v->dudx[i][j] + v->dudx[j][i]);
+            strain += pow(sij, 2);
+        }
+        divu += v->dudx[i][i];
+    }
+    strain = sqrt(2.0 * strain);
+    for (int i = 0; i < D; i++) {
+        for (int j = 0; j < D; j++) {
+            sij = 0.5 * (v->dudx[i][j] + v->dudx[j][i]);
+            v->tauMC[i][j] -= smag * strain * sij;
+        }
+        v->tauMC[i][i] += smag * strain * divu / 3.0;
+    }
+}
+
+static void calc_VIS_Smagorinsky(oracle *o)
+{
+    const int D = o->D;
+    for (int c = 0; c < o->nc; c++) {
+        cellgeo *G = &o->g[c];
+        solvars *vars = &o->v[c];
+        double cR = compute_Rpsi(o, vars->q);
+        double smag = -2.0 * pow(0.16 * pow(1.0 / vars->vol_inv, 1. / 3.), 2);
+        for (int f = 0; f < G->nvalid; f++) {
+            solvars *nb = &o->v[G->neigh[f]];
+            double aR = compute_Rpsi(o, nb->q);
+            double cri = 1.0 / vars->q[0], ari = 1.0 / nb->q[0];
+            double w = G->w[f];
+            double fU[3], cs[3], as[3], cU[3], aU[3];
+            for (int k = 0; k < D; k++) fU[k] = INTERP_LINEAR(w, vars->q[k + 1] * cri, nb->q[k + 1] * ari);
+            double fR = INTERP_LINEAR(w, cR, aR);
+            double fT = fR * o->R_inv;
+            for (int k = 0; k < D; k++) { cs[k] = G->S[f][k] * vars->vol_inv; as[k] = -G->S[f][k] * nb->vol_inv; }
+            for (int i = 0; i < D; i++)
+                for (int j = 0; j < D; j++) {
+                    vars->dudx[i][j] += fU[i] * cs[j];
+                    nb->dudx[i][j] += fU[i] * as[j];
+                }
+            for (int i = 0; i < D; i++) {
+                vars->dTdx[i] += fT * cs[i];
+                nb->dTdx[i] += fT * as[i];
+            }
+            for (int k = 0; k < D; k++) { cU[k] = vars->q[k + 1] / vars->q[0]; aU[k] = nb->q[k + 1] / nb->q[0]; }
+            vis_refresh_smag(o, vars, cU, smag);
+            vis_refresh_smag(o, nb, aU, smag);
+        }
+    }
+}
+
 /* viscous face contribution shared by M1/M2/M2AUSM (cfd_v0.cpp:2682-2790) */
 static void viscous_face(const oracle *o, const cellgeo *G, int f, const solvars *cell, const solvars *nb,
                          const double *cU, const double *aU, double cT, double aT, double S_mag, double *rhs)
@@ -1194,7 +1255,8 @@ static void run_stage(oracle *o, int rk)
         if (o->solver == 2) calc_gradients_M2AUSM(o);
         else calc_gradients(o);
     }
-    calc_VIS(o);
+    if (o->laminar) calc_VIS(o);
+    else calc_VIS_Smagorinsky(o);
     one_rk_step(o, rk, o->dt);
 }
 
@@ -1212,7 +1274,8 @@ int lfmo_stage_phase(void *h, int rk, int begin_step, int phase)
             if (o->solver == 2) calc_gradients_M2AUSM(o);
             else calc_gradients(o);
         }
-        calc_VIS(o);
+        if (o->laminar) calc_VIS(o);
+        else calc_VIS_Smagorinsky(o);
     } else {
         one_rk_step(o, rk, o->dt);
     }

@@ -63,6 +63,8 @@ typedef struct {
     double mol_weight, cp, mu, pr;
     double p_inf, T_inf, U_inf[3];
     double Ls, k_sponge, mach;
+    int laminar;       /* turbulenceProperties simulationType laminar;
+                          0 selects calc_VIS_Smagorinsky            */
 } lfmo_params;
 
 /* Field selectors for lfmo_get_field (all in original cell order). */

@@ -32,7 +32,7 @@ class _Params(C.Structure):
                 ("adjust_dt", C.c_int), ("have_res", C.c_int), ("max_co", C.c_double), ("dt", C.c_double),
                 ("mol_weight", C.c_double), ("cp", C.c_double), ("mu", C.c_double), ("pr", C.c_double),
                 ("p_inf", C.c_double), ("T_inf", C.c_double), ("U_inf", C.c_double * 3),
-                ("Ls", C.c_double), ("k_sponge", C.c_double), ("mach", C.c_double)]
+                ("Ls", C.c_double), ("k_sponge", C.c_double), ("mach", C.c_double), ("laminar", C.c_int)]
 
 
 def build(force: bool = False) -> str:
@@ -93,6 +93,7 @@ def _ptr(a):
 
 def _params(params: dict) -> _Params:
     pr = _Params()
+    pr.laminar = 1
     for k, v in params.items():
         if k == "U_inf":
             for i in range(3):
@@ -160,6 +161,7 @@ class Oracle:
         m.face_key = _ptr(arr(fk, np.int32)) if fk is not None else None
         m.rank = int(getattr(mesh, "rank", -1))
         pr = _Params()
+        pr.laminar = 1
         for k, v in params.items():
             if k == "U_inf":
                 for i in range(3):

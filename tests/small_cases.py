@@ -4,7 +4,8 @@ Each case is small enough for the oracle to finish in well under a second
 per step, and together they cover every branch of the time step: M1 / M2 /
 M2AUSM, inviscid and viscous, walls / inlets / outlets / unset patches,
 cyclic pairs, sponge, mixed quads and triangles, 2D and 3D, every RK order,
-constant and adjustable dt, residuals on and off."""
+constant and adjustable dt, residuals on and off, laminar and Smagorinsky
+LES viscous terms."""
 from __future__ import annotations
 
 import math
@@ -68,4 +69,7 @@ PARITY_CASES = {
     "box_M2AUSM_visc": lambda: with_params(seeded_perturbation(box(6, solver=2, mu=1e-3)), minmod=1),
     "ogrid_M2AUSM_cyclic": lambda: ogrid(6, 12, 4, solver=2, minmod=1),
     "ogrid_M1_cyclic_rk2": lambda: ogrid(6, 12, 4, solver=0, minmod=0, rk_order=2),
+    # simulationType LES: calc_VIS_Smagorinsky
+    "vortex_M2_les": lambda: vortex(24, solver=1, mu=0.05, laminar=0),
+    "box_M2AUSM_les": lambda: with_params(seeded_perturbation(box(6, solver=2, mu=1e-3)), minmod=1, laminar=0),
 }

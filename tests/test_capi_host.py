@@ -2,8 +2,8 @@
 
 The library must load, export every entry point include/lfm_hip.h declares,
 and reject bad configurations before it touches a device, with the
-reference's rules (patch roles, solver/rkOrder/dimension ranges, laminar
-only) and a message from lfm_last_error()."""
+reference's rules (patch roles, solver/rkOrder/dimension ranges) and a
+message from lfm_last_error()."""
 import os
 import re
 
@@ -45,6 +45,7 @@ def test_patch_roles_follow_the_reference_rule():
     assert capi.patch_role("wall", "anything") == meshgen.BC_WALL
     assert capi.patch_role("empty", "frontAndBack") == meshgen.BC_EMPTY
     assert capi.patch_role("cyclic", "front") == meshgen.BC_CYCLIC
+    assert capi.patch_role("processor", "procBoundary0to1") == meshgen.BC_PROCESSOR
     assert capi.patch_role("patch", "INLET") == meshgen.BC_UNSET
     assert capi.patch_role("symmetryPlane", "inlet") == meshgen.BC_UNSET
     assert capi.patch_role("farfield", "far") == meshgen.BC_UNSET
@@ -54,7 +55,6 @@ def test_patch_roles_follow_the_reference_rule():
     ({"solver": 3}, "solver"),
     ({"rk_order": 6}, "rkOrder"),
     ({"dim": 4}, "dimension"),
-    ({"laminar": 0}, "laminar"),
 ])
 def test_bad_parameters_rejected_before_the_device(change, msg):
     case = sc.vortex(8)
