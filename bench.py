@@ -160,7 +160,8 @@ def cpu_baseline(case, budget_s, threads, what="the full mesh"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the configuration's own step count)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="cylinder2d")
     ap.add_argument("--n-theta", type=int, default=1024)
@@ -172,6 +173,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"cylinder2d": 2000, "vortex2d": 1000, "ogrid3d": 500, "box3d": 1000}[args.workload]
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards by spatial "
               f"decomposition but only the single-GPU partition is built and measured.", file=sys.stderr)

@@ -99,7 +99,8 @@ def isentropic_vortex(nx=256, ny=None, L=10.0, strength=5.0, radius=1.0, mach=0.
 
 
 def cylinder_2d(n_theta=512, n_quad=None, n_tri=None, mach=0.3, noise=1e-4, seed=202, solver=2, minmod=1,
-                Ls_diam=5.0, Re=150.0, r_out=None, jitter=0.1, rk_order=5, cfl=0.5, mesh=None) -> Case:
+                Ls_diam=5.0, Re=150.0, r_out=None, jitter=0.1, rk_order=5, cfl=0.5, mesh=None,
+                speed_factor=1.0) -> Case:
     """Unstructured annulus (quads next to the wall, triangles outside) around a
     unit-diameter cylinder, uniform inflow at ``mach`` with seeded pressure noise,
     sponge of length ``Ls_diam`` diameters measured from the outlet patch."""
@@ -138,7 +139,9 @@ def cylinder_2d(n_theta=512, n_quad=None, n_tri=None, mach=0.3, noise=1e-4, seed
     # the explicit viscous limit nu dt / h^2 <= 1/4
     h_min = (1.0 - jitter) * 0.5 * 2 * math.pi / n_theta
     nu = prm["mu"] / rho_inf
-    prm["dt"] = cfl * min(h_min / (U_inf + c_inf), 0.25 * h_min * h_min / nu)
+    # (speed_factor scales U_inf to the fastest flow expected: 2 for the
+    # potential-flow speed at the cylinder's shoulders)
+    prm["dt"] = cfl * min(h_min / (speed_factor * U_inf + c_inf), 0.25 * h_min * h_min / nu)
     del r
     return Case(f"cylinder2d_{nc}", mesh, prm, p, T, U, alpha)
 
