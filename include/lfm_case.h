@@ -63,6 +63,18 @@ int lfm_case_dict_scalar(lfm_case *c, const char *dict, const char *subdict, con
  * q is SoA [(D+2)][n_cells] conservatives in original cell order. */
 int lfm_case_write_time(lfm_case *c, double time, const double *q, int32_t dim);
 
+/* The post-processing controls of controlDict lfm/post (haveAverage,
+ * tStartAverage, haveForces, saveForcesStep; src/inputReader.cpp:19-23) as an
+ * lfm_post for the start time: average_start = (int)((tStartAverage -
+ * startTime) / deltaT) as Mesh::solve computes it (mesh_solver.cpp:253-254),
+ * forces_own_face = 0. */
+int lfm_case_post(lfm_case *c, lfm_post *post);
+
+/* Write one scalar field (n_cells values, original cell order) into the time
+ * directory, boundaries "calculated" (createScalarField fields such as pAvg
+ * and pRMS, mesh_solver.cpp:266-271). */
+int lfm_case_write_scalar(lfm_case *c, double time, const char *name, const char *dimensions, const double *v);
+
 #ifdef __cplusplus
 }
 #endif

@@ -226,6 +226,49 @@ int lfm_part_group_synchronize(lfm_part_group *g);
 int lfm_part_group_destroy(lfm_part_group *g);
 
 /* ---------------------------------------------------------------------
+ * Post-processing on the device: running pressure averages and wall forces
+ * (ISolver::prePostProc / postProcAverage / updateAverageField /
+ * postProcForces, cfd_v0.cpp:2966-2970, 3056-3250, driven as Mesh::solve
+ * drives them, mesh_solver.cpp:253-271, 333-358, 423-429, 740-756, 827-838).
+ * Once configured, lfm_run and lfm_part_group_run do after every step what
+ * Mesh::solve does: time += deltaT (the new deltaT when adjusting),
+ * time_step++, postProcAverage(time_step - average_start), and a force row
+ * when time_step % forces_every == 0.
+ * --------------------------------------------------------------------- */
+typedef struct {
+    int32_t have_average;    /* controlDict lfm/post/haveAverage                 */
+    int32_t average_start;   /* indexStartAverage = (int)((tStartAverage - time)
+                                / deltaT), mesh_solver.cpp:254                    */
+    int32_t have_forces;     /* lfm/post/haveForces                              */
+    int32_t forces_every;    /* lfm/post/saveForcesStep (>= 1)                   */
+    int32_t forces_own_face; /* 0: each wall entry reads the cell slot its face
+                                had before reorder_faces, in the arrays as they
+                                are after it (the reference's indexing); 1: the
+                                wall face's own S, d and ghost                   */
+    double time;             /* current time; the clock of the recorded rows     */
+} lfm_post;
+/* (Re)start post-processing: accumulators zeroed, step counter and clock
+ * reset; have_* = 0 for both turns it off. */
+int lfm_post_configure(lfm_solver *s, const lfm_post *cfg);
+/* ISolver::postProcAverage(n): for n > 0, pAVG += p and pRMS += (p - pAVG/n)^2
+ * per cell (FP64 accumulators on either precision path). */
+int lfm_post_average(lfm_solver *s, int32_t n);
+/* updateAverageField(n): pAvg = pAVG/n, pRMS = sqrt(pRMS/n), original cell
+ * order (n > 0). */
+int lfm_post_average_fields(lfm_solver *s, int32_t n, double *pavg, double *prms);
+/* Wall patches (m_nWallBCList: patches of type wall, in patch order). */
+int lfm_post_walls(lfm_solver *s, int32_t *n, int32_t *patches);
+/* ISolver::postProcForces(time): record one row now.  The boundary ghosts are
+ * read as the last boundary fill left them (after a step, the start of its
+ * last stage); a state set from outside is filled first. */
+int lfm_post_forces(lfm_solver *s, double time);
+/* Drain up to max_rows recorded rows into rows (NULL: *n_rows = rows
+ * pending, nothing drained).  Row layout: time, then per wall patch Fp[D]
+ * and Fv[D] of this partition (a decomposed case sums the partitions, as
+ * postProcForces' MPI_Allreduce does).  Rows are kept in recording order. */
+int lfm_post_forces_read(lfm_solver *s, int32_t max_rows, double *rows, int32_t *n_rows);
+
+/* ---------------------------------------------------------------------
  * Shared launches over several resident solvers (no counterpart in the
  * reference, which runs one solve per process): the FP32-perturbation
  * solvers of one scheme class (dimension, solver, minmod, rkOrder; constant

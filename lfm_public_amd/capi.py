@@ -32,7 +32,9 @@ EXPORTS = ("lfm_patch_role", "lfm_solver_create", "lfm_solver_destroy", "lfm_las
            "lfm_halo_neighbours", "lfm_halo_export", "lfm_halo_import", "lfm_halo_sizes", "lfm_halo_pack",
            "lfm_halo_unpack", "lfm_part_group_create", "lfm_part_group_run", "lfm_part_group_synchronize",
            "lfm_part_group_destroy", "lfm_case_open", "lfm_case_close", "lfm_case_mesh", "lfm_case_params",
-           "lfm_case_fields", "lfm_case_time", "lfm_case_dict_scalar", "lfm_case_write_time")
+           "lfm_case_fields", "lfm_case_time", "lfm_case_dict_scalar", "lfm_case_write_time",
+           "lfm_case_post", "lfm_case_write_scalar", "lfm_post_configure", "lfm_post_average",
+           "lfm_post_average_fields", "lfm_post_walls", "lfm_post_forces", "lfm_post_forces_read")
 HALO_SOLVARS, HALO_VISCOUS, HALO_PACKED = 1, 2, 3
 
 
@@ -61,6 +63,11 @@ class Params(C.Structure):
                 ("p_inf", C.c_double), ("T_inf", C.c_double), ("U_inf", C.c_double * 3),
                 ("Ls", C.c_double), ("k_sponge", C.c_double), ("mach", C.c_double),
                 ("precision", C.c_int32), ("laminar", C.c_int32)]
+
+
+class Post(C.Structure):
+    _fields_ = [("have_average", C.c_int32), ("average_start", C.c_int32), ("have_forces", C.c_int32),
+                ("forces_every", C.c_int32), ("forces_own_face", C.c_int32), ("time", C.c_double)]
 
 
 class Info(C.Structure):
@@ -119,6 +126,12 @@ def load() -> C.CDLL:
     L.lfm_batch_set.argtypes = [vp, vp, C.c_int]
     L.lfm_batch_run.argtypes = [vp, C.c_int]
     L.lfm_batch_synchronize.argtypes = [vp]
+    L.lfm_post_configure.argtypes = [vp, C.POINTER(Post)]
+    L.lfm_post_average.argtypes = [vp, i32]
+    L.lfm_post_average_fields.argtypes = [vp, i32, vp, vp]
+    L.lfm_post_walls.argtypes = [vp, C.POINTER(i32), vp]
+    L.lfm_post_forces.argtypes = [vp, dbl]
+    L.lfm_post_forces_read.argtypes = [vp, i32, vp, C.POINTER(i32)]
     _lib = L
     return L
 
@@ -324,6 +337,48 @@ class GpuSolver:
         n = C.c_int64()
         _check(load().lfm_check_finite(self.h, C.byref(n)))
         return n.value
+
+    # post-processing ------------------------------------------------------
+    def post_configure(self, have_average=False, average_start=0, have_forces=False, forces_every=1,
+                       forces_own_face=False, time=0.0, cfg: Optional[Post] = None):
+        if cfg is None:
+            cfg = Post(int(have_average), int(average_start), int(have_forces), int(forces_every),
+                       int(forces_own_face), float(time))
+        _check(load().lfm_post_configure(self.h, C.byref(cfg)))
+
+    def post_average(self, n: int):
+        """postProcAverage(n)."""
+        _check(load().lfm_post_average(self.h, int(n)))
+
+    def average_fields(self, n: int):
+        """(pAvg, pRMS) in original cell order (updateAverageField(n))."""
+        pavg = np.zeros(self.n_cells)
+        prms = np.zeros(self.n_cells)
+        _check(load().lfm_post_average_fields(self.h, int(n), _ptr(pavg), _ptr(prms)))
+        return pavg, prms
+
+    def walls(self) -> np.ndarray:
+        n = C.c_int32()
+        _check(load().lfm_post_walls(self.h, C.byref(n), None))
+        out = np.zeros(max(n.value, 1), np.int32)
+        _check(load().lfm_post_walls(self.h, C.byref(n), _ptr(out)))
+        return out[: n.value]
+
+    def post_forces(self, time: float):
+        """postProcForces(time): record one row now."""
+        _check(load().lfm_post_forces(self.h, float(time)))
+
+    def forces_read(self):
+        """Drain the recorded rows: (times [n], forces [n, n_walls, 2, D])."""
+        L = load()
+        n = C.c_int32()
+        _check(L.lfm_post_forces_read(self.h, 0, None, C.byref(n)))
+        nw = len(self.walls())
+        width = 1 + 2 * self.dim * nw
+        rows = np.zeros((max(n.value, 1), width))
+        _check(L.lfm_post_forces_read(self.h, n.value, _ptr(rows), C.byref(n)))
+        rows = rows[: n.value]
+        return rows[:, 0].copy(), rows[:, 1:].reshape(n.value, nw, 2, self.dim)
 
     # halo (processor patches) ----------------------------------------------
     def halo_neighbours(self):

@@ -686,6 +686,19 @@ static void write_field(const std::string &path, const char *cls, const char *na
     std::fclose(f);
 }
 
+// boundaryField of a derived field: calculated on every patch (empty and
+// coupled patches keep their type)
+static std::string calculated_boundary(const lfm_case &c) {
+    std::string bnd = "{\n";
+    for (size_t i = 0; i < c.patch_name.size(); i++) {
+        const std::string &t = c.patch_type[i];
+        const bool keep = t == "empty" || t == "cyclic" || t == "processor";
+        bnd += "    " + c.patch_name[i] + "\n    {\n        type            " + (keep ? t : "calculated") +
+               ";\n" + (keep ? "" : "        value           uniform 0;\n") + "    }\n";
+    }
+    return bnd + "}";
+}
+
 extern "C" {
 
 int lfm_case_open(const char *case_dir, int32_t rank, lfm_case **out) {
@@ -777,6 +790,40 @@ int lfm_case_dict_scalar(lfm_case *c, const char *dict, const char *subdict, con
     return LFM_OK;
 }
 
+int lfm_case_post(lfm_case *c, lfm_post *post) {
+    if (!c || !post) return fail(LFM_EINVAL, "null argument");
+    std::memset(post, 0, sizeof *post);
+    try {
+        DictFile control = load_dict(c->root + "/system/controlDict");
+        // CInputReader::initialize, src/inputReader.cpp:19-23
+        post->have_average = dict_bool(control, "lfm/post", "haveAverage", false, true);
+        const double t_avg = dict_scalar(control, "lfm/post", "tStartAverage", 0, true);
+        post->have_forces = dict_bool(control, "lfm/post", "haveForces", false, true);
+        post->forces_every = (int32_t)dict_scalar(control, "lfm/post", "saveForcesStep", 1, true);
+        // Mesh::solve, mesh_solver.cpp:249-254: int indexStartAverage =
+        // (tStartAverage - time) / dt, truncated toward zero
+        post->average_start = (int32_t)((t_avg - c->start) / c->dt);
+    } catch (const ParseError &e) {
+        return fail(LFM_EINVAL, e.msg);
+    }
+    post->time = c->start;
+    return LFM_OK;
+}
+
+int lfm_case_write_scalar(lfm_case *c, double time, const char *name, const char *dimensions, const double *v) {
+    if (!c || !name || !v) return fail(LFM_EINVAL, "null argument");
+    const std::string d = c->dir + "/" + time_name(time);
+    if (!is_dir(d) && mkdir(d.c_str(), 0755) != 0) return fail(LFM_EINVAL, d + ": " + std::strerror(errno));
+    try {
+        std::vector<double> vals(v, v + c->n_cells);
+        write_field(d + "/" + name, "volScalarField", name, dimensions ? dimensions : "", vals, 1,
+                    calculated_boundary(*c));
+    } catch (const ParseError &e) {
+        return fail(LFM_EINVAL, e.msg);
+    }
+    return LFM_OK;
+}
+
 int lfm_case_write_time(lfm_case *c, double time, const double *q, int32_t dim) {
     if (!c || !q || (dim != 2 && dim != 3)) return fail(LFM_EINVAL, "bad argument");
     const size_t n = (size_t)c->n_cells;
@@ -803,15 +850,7 @@ int lfm_case_write_time(lfm_case *c, double time, const double *q, int32_t dim) 
         write_field(d + "/p", "volScalarField", "p", c->fp.dimensions, p, 1, c->fp.boundary_text);
         write_field(d + "/T", "volScalarField", "T", c->fT.dimensions, T, 1, c->fT.boundary_text);
         write_field(d + "/U", "volVectorField", "U", c->fU.dimensions, U, 3, c->fU.boundary_text);
-        // derived fields: calculated on every patch (empty / coupled kept)
-        std::string bnd = "{\n";
-        for (size_t i = 0; i < c->patch_name.size(); i++) {
-            const std::string &t = c->patch_type[i];
-            const bool keep = t == "empty" || t == "cyclic" || t == "processor";
-            bnd += "    " + c->patch_name[i] + "\n    {\n        type            " + (keep ? t : "calculated") +
-                   ";\n" + (keep ? "" : "        value           uniform 0;\n") + "    }\n";
-        }
-        bnd += "}";
+        const std::string bnd = calculated_boundary(*c);
         write_field(d + "/rho", "volScalarField", "rho", "[1 -3 0 0 0 0 0]", rho, 1, bnd);
         write_field(d + "/E", "volScalarField", "E", "[0 2 -2 0 0 0 0]", E, 1, bnd);
     } catch (const ParseError &e) {

@@ -304,6 +304,7 @@ std::string build_layout(const lfm_polymesh &m, const lfm_partition *part, const
                             if (ci >= L.nsm0) return "boundary face on an interior-submesh cell";
                             L.ghost_cell.push_back(ci);
                             L.ghost_role.push_back(role);
+                            L.ghost_patch.push_back(pch);
                             acc++;
                         }
                     }
@@ -314,7 +315,9 @@ std::string build_layout(const lfm_polymesh &m, const lfm_partition *part, const
         L.ng = acc;
     }
     L.rank = part ? part->rank : -1;
+    L.patch_role.assign(m.patch_role, m.patch_role + m.n_patches);
     std::vector<int32_t> gcount(m.n_patches + 1, 0);
+    std::vector<int32_t> ghost_slot(L.ng_bnd, -1);  // original slot of each boundary ghost's face
     for (int ci = 0; ci < nc; ci++) {
         const int c = L.orig[ci];
         int s = soff[ci];
@@ -338,6 +341,7 @@ std::string build_layout(const lfm_polymesh &m, const lfm_partition *part, const
                 for (int k = 0; k < D; k++) sl.d[k] = 2.0 * std::fabs(dPS) * sl.S[k] / S2;
                 const int pch = fpatch[f];
                 sl.nb = nc + gstart[pch] + gcount[pch]++;
+                if (sl.nb - nc < L.ng_bnd) ghost_slot[sl.nb - nc] = s - 1;
                 if (m.patch_role[pch] == LFM_BC_PROCESSOR) {
                     // halo ghost; d and w are replaced by halo_import
                     const int rank = m.patch_nbr[pch];
@@ -395,6 +399,24 @@ std::string build_layout(const lfm_polymesh &m, const lfm_partition *part, const
             const int dst = ok ? vi++ : ii--;
             perm[b + dst] = b + j;
             svalid[b + dst] = ok;
+        }
+    }
+
+    // postProcForces' view of each boundary ghost's face
+    L.gf_ref_nb.assign(L.ng_bnd, 0);
+    L.gf_ref_S.assign((size_t)D * L.ng_bnd, 0.0);
+    L.gf_ref_d.assign((size_t)D * L.ng_bnd, 0.0);
+    L.gf_own_S.assign((size_t)D * L.ng_bnd, 0.0);
+    L.gf_own_d.assign((size_t)D * L.ng_bnd, 0.0);
+    for (int g = 0; g < L.ng_bnd; g++) {
+        if (ghost_slot[g] < 0) return "internal error: boundary ghost without a slot";
+        const Slot &own = slots[ghost_slot[g]], &ref = slots[perm[ghost_slot[g]]];
+        L.gf_ref_nb[g] = ref.nb;
+        for (int k = 0; k < D; k++) {
+            L.gf_ref_S[(size_t)k * L.ng_bnd + g] = ref.S[k];
+            L.gf_ref_d[(size_t)k * L.ng_bnd + g] = ref.d[k];
+            L.gf_own_S[(size_t)k * L.ng_bnd + g] = own.S[k];
+            L.gf_own_d[(size_t)k * L.ng_bnd + g] = own.d[k];
         }
     }
 

@@ -60,6 +60,16 @@ struct SolverLayout {
     std::vector<int32_t> s_off, s_ent;
     // boundary ghosts: interior cell (combined index) and BC role
     std::vector<int32_t> ghost_cell, ghost_role;
+    // per boundary ghost: its patch, and the face postProcForces reads for
+    // it (cfd_v0.cpp:3169-3233): boundaries[patch][i] keeps the slot the
+    // face had before reorder_faces, the cell's arrays are indexed after it,
+    // so the face read is whichever moved into that slot (ref), or the
+    // ghost's own face (own).  nb: combined index of the neighbour read;
+    // S, d: [D][ng_bnd] as the cell sees that face.
+    std::vector<int32_t> ghost_patch;
+    std::vector<int32_t> patch_role;  // [n_patches] LFM_BC_* of the polyMesh patches
+    std::vector<int32_t> gf_ref_nb;
+    std::vector<double> gf_ref_S, gf_ref_d, gf_own_S, gf_own_d;
     // initial conservatives [(D+2)][nc] combined order
     std::vector<double> q0;
 };

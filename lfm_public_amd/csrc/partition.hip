@@ -334,6 +334,8 @@ int lfm_part_group_run(lfm_part_group *g, int nsteps) {
         if (rc) break;
         for (Solver *p : g->parts) p->end_step(!g->adjust);
         if (g->adjust) hipLaunchKernelGGL(lfm::group_dt_min, dim3(1), dim3(64), 0, g->stream, g->d_dtmin, g->d_dt, n);
+        for (Solver *p : g->parts)
+            if (p->post && !rc) rc = p->post_step();
     }
     hipError_t e = hipEventRecord(g->done, g->stream);
     for (int p = 0; p < n; p++) {

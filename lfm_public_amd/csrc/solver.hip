@@ -70,6 +70,7 @@ static int dev_zeros(Solver *s, T **dst, size_t n) {
 Solver::~Solver() {
     for (void *p : allocs) (void)hipFree(p);
     if (k32) free_k32(k32);
+    free_post(post);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -316,6 +317,9 @@ void Solver::stage_b(int rk) {
     prof_mark(2, true);
     flux();
     prof_mark(2, false);
+    // the update refreshes the boundary ghosts from the updated cells; the
+    // wall forces after this step read them as this stage began
+    if (post_forces && rk == nstages - 1) post_snapshot();
     prof_mark(3, true);
     update(A[rk], B[rk], rk == 0, res && rk == 0, false);
     prof_mark(3, false);
@@ -464,6 +468,7 @@ int lfm_set_boundary_conditions(lfm_solver *h) {
     if (!h) return fail(LFM_EINVAL, "null handle");
     h->impl->bc();
     h->impl->ghosts_stale = false;
+    if (h->impl->post_forces) h->impl->post_snapshot();
     HIP_TRY(hipGetLastError());
     return LFM_OK;
 }
@@ -535,7 +540,13 @@ int lfm_run(lfm_solver *h, int nsteps) {
     if (!s->L.halo.empty())
         return fail(LFM_ESTATE, "a partition with processor patches steps through lfm_part_group_run or the "
                                 "per-stage calls with halo exchange");
-    for (int i = 0; i < nsteps; i++) s->step();
+    for (int i = 0; i < nsteps; i++) {
+        s->step();
+        if (s->post) {
+            const int rc = s->post_step();
+            if (rc) return rc;
+        }
+    }
     HIP_TRY(hipGetLastError());
     return LFM_OK;
 }

@@ -17,6 +17,9 @@ namespace lfm {
 
 // FP32-perturbation kernel arguments (defined in kernels_fp32p.hpp).
 struct KArgs32;
+// Post-processing state (averages, wall forces; defined in post.hip).
+struct PostState;
+void free_post(PostState *p);
 
 int fail(int code, const std::string &msg);
 
@@ -81,6 +84,16 @@ struct Solver {
     int halo_pack(int ni, int kind, void *buf);
     int halo_unpack(int ni, int kind, const void *buf);
     int halo_upload_geometry();  // d, w of the faces after an import
+
+    // post-processing (lfm_post_configure); post_forces mirrors
+    // post->cfg.have_forces for the stage code
+    PostState *post = nullptr;
+    bool post_forces = false;
+    // copies the boundary ghosts the wall forces read (their state as the
+    // current stage's boundary fill left it)
+    void post_snapshot();
+    // the Mesh::solve post-processing after one step (time, averages, forces)
+    int post_step();
 
     bool ghosts_stale = true;  // ghost states not derived from the current q
     double pending_A = 1.0;

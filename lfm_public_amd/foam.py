@@ -102,15 +102,20 @@ def write_fields(d: str, time_name: str, mesh: meshgen.PolyMesh, p, T, U, alpha=
            "".join(f"({_r(a)} {_r(b)} {_r(c)})\n" for a, b, c in Ua) + ")\n;\n\n" + bnd)
 
 
-def write_dicts(d: str, params: dict, end_time: float, write_interval: int = 100, extra_control: str = ""):
+def write_dicts(d: str, params: dict, end_time: float, write_interval: int = 100, extra_control: str = "",
+                post: Optional[dict] = None):
+    """system/ and constant/ dictionaries; ``post`` holds lfm/post keys
+    (haveAverage, tStartAverage, haveForces, saveForcesStep, ...)."""
     prm = params
     yes = lambda b: "true" if b else "false"  # noqa: E731
+    post_keys = "".join(f"        {k} {yes(v) if isinstance(v, bool) else _r(v) if isinstance(v, float) else v};\n"
+                        for k, v in (post or {}).items())
     _write(os.path.join(d, "system", "controlDict"),
            _HEADER.format(cls="dictionary", loc="system", obj="controlDict") +
            "lfm\n{\n    commType 2;\n    haloCommType 1;\n"
            f"    doublePrecision {yes(prm.get('precision', 64) == 64)};\n"
            f"    post\n    {{\n        haveResiduals {yes(prm.get('have_res', 0))};\n"
-           "        printInfoFreq 10;\n    }\n}\n\n"
+           "        printInfoFreq 10;\n" + post_keys + "    }\n}\n\n"
            f"startFrom       startTime;\nstartTime       0;\nstopAt          endTime;\n"
            f"endTime         {_r(end_time)};\ndeltaT          {_r(prm['dt'])};\nwriteControl    timeStep;\n"
            f"writeInterval   {int(write_interval)};\nwriteFormat     ascii;\nwritePrecision  17;\n"
@@ -139,11 +144,11 @@ def write_dicts(d: str, params: dict, end_time: float, write_interval: int = 100
 
 
 def write_case(d: str, case: Case, end_steps: int = 100, write_interval: int = 100,
-               cell_rank: Optional[np.ndarray] = None):
+               cell_rank: Optional[np.ndarray] = None, post: Optional[dict] = None):
     """The case as an OpenFOAM directory; with cell_rank, decomposed into
     processor<N>/ (polyMesh with processor patches and faceProcAddressing,
     0/ fields) beside the root's system/ and constant/ dictionaries."""
-    write_dicts(d, case.params, end_steps * case.params["dt"], write_interval)
+    write_dicts(d, case.params, end_steps * case.params["dt"], write_interval, post=post)
     if cell_rank is None:
         write_polymesh(d, case.mesh)
         write_fields(d, "0", case.mesh, case.p, case.T, case.U, case.alpha)
@@ -172,6 +177,8 @@ def _lib():
         L.lfm_case_dict_scalar.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_double, C.c_int32,
                                            C.POINTER(C.c_double)]
         L.lfm_case_write_time.argtypes = [vp, C.c_double, vp, C.c_int32]
+        L.lfm_case_post.argtypes = [vp, C.POINTER(capi.Post)]
+        L.lfm_case_write_scalar.argtypes = [vp, C.c_double, C.c_char_p, C.c_char_p, vp]
         L._case_bound = True
     return L
 
@@ -247,6 +254,18 @@ class FoamCase:
     def write_time(self, time: float, q: np.ndarray, dim: int):
         qa = np.ascontiguousarray(q, np.float64)
         capi._check(_lib().lfm_case_write_time(self.h, time, qa.ctypes.data, dim))
+
+    @property
+    def post(self) -> capi.Post:
+        """lfm/post controls (haveAverage, tStartAverage, haveForces,
+        saveForcesStep) for the start time."""
+        out = capi.Post()
+        capi._check(_lib().lfm_case_post(self.h, C.byref(out)))
+        return out
+
+    def write_scalar(self, time: float, name: str, values: np.ndarray, dimensions: str = ""):
+        v = np.ascontiguousarray(values, np.float64)
+        capi._check(_lib().lfm_case_write_scalar(self.h, time, name.encode(), dimensions.encode(), v.ctypes.data))
 
     def close(self):
         if getattr(self, "h", None):

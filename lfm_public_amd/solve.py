@@ -5,7 +5,11 @@ for what the time step needs: read the case (undecomposed, or every
 processor<N>/ of a decomposed one), create the resident solver(s), advance
 from startTime to endTime, print step / time / deltaT / CFL (and the
 residuals when lfm/post/haveResiduals is on) every printInfoFreq steps, and
-write p, U, T, rho, E into time directories at the writeInterval.
+write p, U, T, rho, E into time directories at the writeInterval.  With
+lfm/post/haveAverage the running pAvg / pRMS fields are written beside them;
+with lfm/post/haveForces the wall forces recorded every saveForcesStep steps
+are appended to output/postProcess_Forces_<k> (one file per wall patch,
+mesh_solver.cpp:333-358, 1081-1123; cfd_v0.cpp:3255-3280).
 
     python -m lfm_public_amd.solve CASE [--steps N] [--precision 64|32]
 """
@@ -13,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import math
+import os
 import sys
 import time
 
@@ -23,6 +28,26 @@ from . import capi, foam
 
 def _steps_between(start, end, dt):
     return max(0, int(round((end - start) / dt)))
+
+
+_XYZ = "xyz"
+
+
+def _forces_header(path: str, dim: int):
+    """output_Forces(firstRun), mesh_solver.cpp:1104-1117."""
+    cols = f"{'time':>25}" + "".join(f"{'Fp,':>25}{_XYZ[k]}" for k in range(dim))
+    cols += "".join(f"{'Fv,':>25}{_XYZ[k]}" for k in range(dim))
+    cols += "".join(f"{'Fp,':>25}{_XYZ[k]} + Fv,{_XYZ[k]}" for k in range(dim))
+    with open(path, "w") as f:
+        f.write(cols + "\n")
+
+
+def _forces_append(path: str, times, F):
+    """postProcForcesOutput, cfd_v0.cpp:3255-3272: time, Fp, Fv, Fp + Fv."""
+    with open(path, "a") as f:
+        for t, (fp, fv) in zip(times, F):
+            vals = [t, *fp, *fv, *(fp + fv)]
+            f.write("".join(f"{v:25.16e}" for v in vals) + "\n")
 
 
 def run_case(case_dir: str, steps=None, precision=None, quiet=False, write=True) -> dict:
@@ -37,6 +62,31 @@ def run_case(case_dir: str, steps=None, precision=None, quiet=False, write=True)
         capi.connect(solvers)
         group = capi.PartGroup(solvers)
     info_freq = int(c0.dict_scalar("system/controlDict", "lfm/post", "printInfoFreq", 1)) or 1
+    post = c0.post
+    forces_files = []
+    if post.have_average or post.have_forces:
+        for s in solvers:
+            s.post_configure(cfg=post)
+    if post.have_forces:
+        walls = solvers[0].walls()
+        out_dir = os.path.join(case_dir, "output")
+        os.makedirs(os.path.join(out_dir, "Times"), exist_ok=True)
+        forces_files = [os.path.join(out_dir, f"postProcess_Forces_{k + 1}") for k in range(len(walls))]
+        if write and forces_files and not os.path.exists(forces_files[0]):
+            for fn in forces_files:
+                _forces_header(fn, dim)
+        for s in solvers:  # the start-up postProcForces (mesh_solver.cpp:427-428)
+            s.post_forces(c0.start)
+
+    def flush_forces():
+        rows = [s.forces_read() for s in solvers]
+        times = rows[0][0]
+        F = sum(r[1] for r in rows)  # MPI_Allreduce SUM over ranks
+        if write:
+            for k, fn in enumerate(forces_files):
+                _forces_append(fn, times, F[:, k])
+        return times, F
+    forces_log = []
     n_total = steps if steps is not None else _steps_between(c0.start, c0.end, c0.delta_t)
     adjust = bool(prm["adjust_dt"])
     t = c0.start
@@ -78,13 +128,25 @@ def run_case(case_dir: str, steps=None, precision=None, quiet=False, write=True)
         if write and (due or done == n_total):
             for c, s in zip(cases, solvers):
                 c.write_time(t, s.q(), dim)
+                if post.have_average:
+                    n = done - post.average_start
+                    pavg, prms = s.average_fields(n) if n > 0 else (np.zeros(s.n_cells), np.zeros(s.n_cells))
+                    c.write_scalar(t, "pAvg", pavg, "[1 -1 -2 0 0 0 0]")
+                    c.write_scalar(t, "pRMS", prms, "[1 -1 -2 0 0 0 0]")
+            if post.have_forces:
+                forces_log.append(flush_forces())
             written.append(t)
             if due:
                 next_write += c0.write_interval
     wall = time.perf_counter() - t0
     ncells = sum(c.mesh_arrays["n_cells"] for c in cases)
+    if post.have_forces:
+        forces_log.append(flush_forces())
     out = dict(steps=done, time=t, wall_s=wall, cell_updates_per_s=ncells * done / max(wall, 1e-12),
                written=written, q=[s.q() for s in solvers])
+    if post.have_forces:
+        out["forces_time"] = np.concatenate([x[0] for x in forces_log])
+        out["forces"] = np.concatenate([x[1] for x in forces_log])
     if group is not None:
         group.close()
     for s in solvers:

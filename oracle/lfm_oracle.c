@@ -80,6 +80,11 @@ typedef struct {
     /* ghost bookkeeping */
     int *ghost_cell;      /* [ng] interior combined index              */
     int *ghost_role;      /* [ng]                                      */
+    int *ghost_patch;     /* [ng] patch of the ghost's face              */
+    int *ghost_slot;      /* [ng] slot of that face in its cell BEFORE
+                             reorder_faces (boundaries[ib][i][1])        */
+    int n_walls, *wall_patch;  /* m_nWallBCList, patch order           */
+    double *pAVG, *pRMS;  /* [nc] postProcAverage accumulators         */
     double *centre, *volume;
     /* physics */
     double Ak[5], Bk[5];
@@ -368,6 +373,12 @@ void *lfmo_create(const lfmo_mesh *m, const lfmo_params *prm,
     o->g = calloc((size_t)nc, sizeof(cellgeo));
     o->ghost_cell = malloc(sizeof(int) * (ng + 1));
     o->ghost_role = malloc(sizeof(int) * (ng + 1));
+    o->ghost_patch = malloc(sizeof(int) * (ng + 1));
+    o->ghost_slot = malloc(sizeof(int) * (ng + 1));
+    o->wall_patch = malloc(sizeof(int) * (m->n_patches + 1));
+    o->n_walls = 0;
+    for (int pch = 0; pch < m->n_patches; pch++)
+        if (m->patch_role[pch] == LFMO_BC_WALL) o->wall_patch[o->n_walls++] = pch;
     o->centre = cc;
     o->volume = vol;
 
@@ -462,7 +473,10 @@ void *lfmo_create(const lfmo_mesh *m, const lfmo_params *prm,
                     for (int k = 0; k < D; k++) S2 += G->S[s][k] * G->S[s][k];
                     for (int k = 0; k < D; k++) dPS += dP[s][k] * G->S[s][k];
                     for (int k = 0; k < D; k++) G->d[s][k] = 2.0 * fabs(dPS) * G->S[s][k] / S2;
-                    G->neigh[s] = nc + gstart[pch] + gcount[pch]++;
+                    G->neigh[s] = nc + gstart[pch] + gcount[pch];
+                    o->ghost_patch[gstart[pch] + gcount[pch]] = pch;
+                    o->ghost_slot[gstart[pch] + gcount[pch]] = s;
+                    gcount[pch]++;
                     ghostslot[s] = 1;
                 } else {
                     /* v_neigh: mesh_reader.cpp:421-442 (owner-relative), times v_sign */
@@ -585,6 +599,7 @@ fail:
     free(fc); free(fa); free(fpatch); free(valid); free(sm); free(ncf); free(coff); free(cfaces);
     if (o) {
         free(o->orig); free(o->comb); free(o->v); free(o->g); free(o->ghost_cell); free(o->ghost_role);
+        free(o->ghost_patch); free(o->ghost_slot); free(o->wall_patch);
         free(o);
     }
     free(cc); free(vol);
@@ -598,6 +613,7 @@ void lfmo_destroy(void *h)
     free(o->pf_cell); free(o->pf_slot); free(o->pf_key); free(o->pf_xf); free(o->pf_xc);
     for (int b = 0; b < o->nnb; b++) free(o->nb_recv[b]);
     free(o->orig); free(o->comb); free(o->v); free(o->g); free(o->ghost_cell); free(o->ghost_role);
+    free(o->ghost_patch); free(o->ghost_slot); free(o->wall_patch); free(o->pAVG); free(o->pRMS);
     free(o->centre); free(o->volume);
     free(o);
 }
@@ -1548,6 +1564,124 @@ int lfmo_get_slots(void *h, int *nface, int *nvalid, int *nbr, double *S, double
         }
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* post-processing: averages and wall forces                           */
+/* ------------------------------------------------------------------ */
+int lfmo_set_boundary_conditions(void *h)
+{
+    set_boundary_conditions((oracle *)h);
+    return 0;
+}
+
+/* postProcAverage(time_step), cfd_v0.cpp:3083-3103: for n > 0 add the cell
+ * pressure to pAVG and (p - pAVG/n)^2 to pRMS, pAVG already holding p. */
+int lfmo_average(void *h, int n)
+{
+    oracle *o = h;
+    const int D = o->D;
+    if (!o->pAVG) {
+        o->pAVG = calloc((size_t)o->nc, sizeof(double));
+        o->pRMS = calloc((size_t)o->nc, sizeof(double));
+    }
+    if (n <= 0) return 0;
+    for (int ci = 0; ci < o->nc; ci++) {
+        const double *q = o->v[ci].q;
+        double r = q[0], velMag = 0.0;
+        for (int k = 1; k <= D; k++) {
+            double vel = q[k] / r;
+            velMag += vel * vel;
+        }
+        double E = q[D + 1] / r;
+        double p = (r * o->gm1) * (E - 0.5 * velMag);
+        o->pAVG[ci] += p;
+        o->pRMS[ci] += pow(p - o->pAVG[ci] / n, 2);
+    }
+    return 0;
+}
+
+/* updateAverageField(n), cfd_v0.cpp:2966-2970 with the field updates of
+ * polyMeshReaderOF.cpp:792-861: pAvg = pAVG * (1/n), pRMS = sqrt(pRMS * (1/n)),
+ * original cell order. */
+int lfmo_average_fields(void *h, int n, double *pavg, double *prms)
+{
+    oracle *o = h;
+    if (!o->pAVG || n <= 0) return -1;
+    const double f = 1.0 / n;
+    for (int ci = 0; ci < o->nc; ci++) {
+        pavg[o->orig[ci]] = o->pAVG[ci] * f;
+        prms[o->orig[ci]] = sqrt(o->pRMS[ci] * f);
+    }
+    return 0;
+}
+
+int lfmo_walls(void *h, int *patches)
+{
+    oracle *o = h;
+    for (int i = 0; i < o->n_walls && patches; i++) patches[i] = o->wall_patch[i];
+    return o->n_walls;
+}
+
+/* postProcForces, cfd_v0.cpp:3169-3233, per wall patch in m_nWallBCList
+ * order: out[w*2D + k] = Fp_k, out[w*2D + D + k] = Fv_k.  Each wall face is
+ * looked up as the reference does, by boundaries[patch][i] = {cell, slot}
+ * with the slot its face had BEFORE reorder_faces, in the cell's arrays as
+ * they are AFTER it (S, d and neighbour of whatever face moved there).
+ * own_face != 0 uses the wall face's own S, d and ghost instead.  The
+ * neighbour's state is what it holds now: for boundary ghosts, the fill of
+ * the last set_boundary_conditions. */
+int lfmo_forces(void *h, int own_face, double *out)
+{
+    oracle *o = h;
+    const int D = o->D;
+    const double mu = o->mu;
+    for (int w = 0; w < o->n_walls; w++) {
+        double Fpre[3] = {0.0, 0.0, 0.0}, Fvis[3] = {0.0, 0.0, 0.0};
+        for (int g = 0; g < o->ng; g++) {
+            if (o->ghost_patch[g] != o->wall_patch[w]) continue;
+            const int ci = o->ghost_cell[g];
+            const cellgeo *G = &o->g[ci];
+            int face = o->ghost_slot[g];
+            if (own_face)
+                for (face = 0; G->neigh[face] != o->nc + g; face++) {}
+            const solvars *vars = &o->v[ci];
+            const solvars *neigh = &o->v[G->neigh[face]];
+            double r = vars->q[0], Umag = 0.0;
+            for (int k = 0; k < D; k++) {
+                double u = vars->q[k + 1] / r;
+                Umag += u * u;
+            }
+            Umag = sqrt(Umag);
+            double E = vars->q[D + 1] / r;
+            double p = (r * o->gm1) * (E - 0.5 * Umag * Umag);
+            for (int k = 0; k < D; k++) Fpre[k] += G->S[face][k] * (p - o->p_inf);
+            double dmag_inv = 1.0 / magD(D, G->d[face]);
+            double dn[3], dudx[3][3], tau[3][3];
+            for (int k = 0; k < D; k++) dn[k] = G->d[face][k] * dmag_inv;
+            for (int i = 0; i < D; i++) {
+                double cu = vars->q[i + 1] / vars->q[0];
+                double au = neigh->q[i + 1] / neigh->q[0];
+                for (int j = 0; j < D; j++) dudx[i][j] = (au - cu) * dn[j] * dmag_inv;
+            }
+            for (int a = 0; a < D; a++) {
+                tau[a][a] = 2.0 * dudx[a][a];
+                for (int a1 = a + 1; a1 < D + a; a1++) {
+                    int a2 = a1 % D;
+                    tau[a][a2] = mu * (dudx[a][a2] + dudx[a2][a]);
+                    tau[a][a] -= dudx[a2][a2];
+                }
+                tau[a][a] *= 2.0 / 3.0 * mu;
+            }
+            for (int a = 0; a < D; a++)
+                for (int b = 0; b < D; b++) Fvis[a] -= tau[a][b] * G->S[face][b];
+        }
+        for (int k = 0; k < D; k++) {
+            out[w * 2 * D + k] = Fpre[k];
+            out[w * 2 * D + D + k] = Fvis[k];
+        }
+    }
+    return o->n_walls;
 }
 
 /* ------------------------------------------------------------------ */

@@ -123,6 +123,14 @@ int    lfmo_halo_unpack(void *h, int rank, int kind, const double *buf);
 int    lfmo_stage_phase(void *h, int rk, int begin_step, int phase);
 void   lfmo_set_dt(void *h, double dt);
 
+/* Post-processing (cfd_v0.cpp:2966-2970, 3083-3103, 3169-3233). */
+int    lfmo_set_boundary_conditions(void *h);
+int    lfmo_average(void *h, int n);                              /* postProcAverage(n)  */
+int    lfmo_average_fields(void *h, int n, double *pavg, double *prms);
+int    lfmo_walls(void *h, int *patches);                         /* returns the count   */
+/* per wall patch: Fp[D] then Fv[D]; own_face 0 = the reference's slot lookup */
+int    lfmo_forces(void *h, int own_face, double *out);
+
 /* Unit-level pieces (closed-form tests of the oracle itself). */
 double lfmo_minmod(int D, double cphi, double aphi, const double *grad, const double *d, double wl, double flux);
 void   lfmo_p5(double M, double alpha, double *pos_neg);

@@ -81,6 +81,11 @@ def lib():
         L.lfmo_halo_unpack.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
         L.lfmo_stage_phase.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.lfmo_set_dt.argtypes = [C.c_void_p, C.c_double]
+        L.lfmo_set_boundary_conditions.argtypes = [C.c_void_p]
+        L.lfmo_average.argtypes = [C.c_void_p, C.c_int]
+        L.lfmo_average_fields.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.lfmo_walls.argtypes = [C.c_void_p, C.c_void_p]
+        L.lfmo_forces.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.lfmo_face_rhs.argtypes = [C.POINTER(_Params), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                     C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -234,6 +239,34 @@ class Oracle:
     @property
     def n_ghost(self) -> int:
         return lib().lfmo_n_ghost(self.h)
+
+    # post-processing ---------------------------------------------------------
+    def set_boundary_conditions(self):
+        lib().lfmo_set_boundary_conditions(self.h)
+
+    def average(self, n: int):
+        """postProcAverage(n)."""
+        lib().lfmo_average(self.h, int(n))
+
+    def average_fields(self, n: int):
+        pavg = np.zeros(self.n_cells)
+        prms = np.zeros(self.n_cells)
+        if lib().lfmo_average_fields(self.h, int(n), _ptr(pavg), _ptr(prms)) != 0:
+            raise ValueError("no averages accumulated")
+        return pavg, prms
+
+    def walls(self):
+        n = lib().lfmo_walls(self.h, None)
+        out = np.zeros(max(n, 1), np.int32)
+        lib().lfmo_walls(self.h, _ptr(out))
+        return out[:n]
+
+    def forces(self, own_face: bool = False) -> np.ndarray:
+        """postProcForces: [n_walls, 2, D] (pressure, viscous)."""
+        n = len(self.walls())
+        out = np.zeros(max(n, 1) * 2 * self.dim)
+        lib().lfmo_forces(self.h, int(own_face), _ptr(out))
+        return out[: n * 2 * self.dim].reshape(n, 2, self.dim)
 
     # partitions --------------------------------------------------------------
     def halo_neighbours(self):

@@ -73,3 +73,41 @@ PARITY_CASES = {
     "vortex_M2_les": lambda: vortex(24, solver=1, mu=0.05, laminar=0),
     "box_M2AUSM_les": lambda: with_params(seeded_perturbation(box(6, solver=2, mu=1e-3)), minmod=1, laminar=0),
 }
+
+
+def reorder_patches(mesh, order):
+    """The same mesh with its boundary patches renumbered in ``order`` (patch
+    names): boundary faces are regrouped, internal faces untouched.  OpenFOAM
+    accepts any patch order; the order decides where each boundary face sits
+    in its cell's face list."""
+    byname = {p.name: p for p in mesh.patches}
+    assert sorted(order) == sorted(byname)
+    ni = mesh.n_internal
+    faces = list(range(ni))
+    patches = []
+    for name in order:
+        p = byname[name]
+        patches.append(meshgen.Patch(p.name, p.type, len(faces), p.size, p.neighbour, p.neighb_proc))
+        faces.extend(range(p.start, p.start + p.size))
+    faces = np.array(faces)
+    sizes = np.diff(mesh.face_off)[faces]
+    off = np.zeros(len(faces) + 1, np.int32)
+    np.cumsum(sizes, out=off[1:])
+    pts = np.concatenate([mesh.face_pts[mesh.face_off[f]:mesh.face_off[f + 1]] for f in faces]).astype(np.int32)
+    return meshgen.PolyMesh(mesh.points, off, pts, mesh.owner[faces].astype(np.int32), mesh.neighbour, mesh.n_cells,
+                            patches, mesh.dim)
+
+
+def channel(nx=8, ny=6, Lx=2.0, Ly=1.5, thickness=0.5, mu=1e-3, shear=0.0, dp=0.0):
+    """2D channel [0,Lx]x[0,Ly] with walls at y=0 ("bottom") and y=Ly ("top"),
+    inlet/outlet at x=0/Lx; U = (shear*y, 0), p = p_inf + dp, T = T_inf."""
+    mesh = meshgen.cartesian_2d(nx, ny, 0.0, Lx, 0.0, Ly,
+                                names=(("inlet", "patch"), ("outlet", "patch"), ("bottom", "wall"),
+                                       ("top", "wall")), thickness=thickness)
+    prm = cases.base_params(dim=2, solver=1, mu=mu)
+    cc = meshgen.cell_centres_approx(mesh)
+    U = np.zeros((mesh.n_cells, 3))
+    U[:, 0] = shear * cc[:, 1]
+    p = np.full(mesh.n_cells, prm["p_inf"] + dp)
+    T = np.full(mesh.n_cells, prm["T_inf"])
+    return cases.Case("channel", mesh, prm, p, T, U)

@@ -185,3 +185,36 @@ def test_binary_mesh_is_rejected(tmp_path):
     open(pts, "w").write(open(pts).read().replace("format      ascii", "format      binary"))
     with pytest.raises(capi.LfmError, match="binary"):
         foam.FoamCase(d)
+
+
+def test_post_controls(tmp_path):
+    """lfm/post haveAverage / tStartAverage / haveForces / saveForcesStep
+    (src/inputReader.cpp:19-23) and indexStartAverage = (int)((tStartAverage
+    - startTime) / deltaT) (mesh_solver.cpp:253-254), with the defaults."""
+    c = sc.channel()
+    dt = c.params["dt"]
+    d = str(tmp_path / "post")
+    foam.write_case(d, c, end_steps=10, post=dict(haveAverage=True, tStartAverage=3.5 * dt, haveForces=True,
+                                                  saveForcesStep=4))
+    p = foam.FoamCase(d).post
+    assert (p.have_average, p.average_start, p.have_forces, p.forces_every, p.forces_own_face) == (1, 3, 1, 4, 0)
+    assert p.time == 0.0
+    d2 = str(tmp_path / "nopost")
+    foam.write_case(d2, c, end_steps=10)
+    p = foam.FoamCase(d2).post
+    assert (p.have_average, p.average_start, p.have_forces, p.forces_every) == (0, 0, 0, 1)
+
+
+def test_write_scalar(tmp_path):
+    c = sc.channel()
+    d = str(tmp_path / "ws")
+    foam.write_case(d, c, end_steps=10)
+    fc = foam.FoamCase(d)
+    v = np.linspace(1.0, 2.0, c.mesh.n_cells) / 3.0
+    fc.write_scalar(0.25, "pAvg", v, "[1 -1 -2 0 0 0 0]")
+    txt = open(os.path.join(d, "0.25", "pAvg")).read()
+    assert "class       volScalarField;" in txt and "object      pAvg;" in txt
+    body = txt.split("internalField   nonuniform List<scalar>")[1].split("(", 1)[1].split(")", 1)[0]
+    assert np.array_equal(np.array(body.split(), float), v)
+    assert "bottom\n    {\n        type            calculated;" in txt
+    assert "frontAndBack" not in txt or "type            empty" in txt

@@ -53,3 +53,48 @@ def test_solve_driver_serial_and_decomposed(tmp_path):
     # the written time directory holds p of the final state
     tname = repr(12 * c.params["dt"])
     assert os.path.isdir(os.path.join(ds, "%.12g" % (12 * c.params["dt"]))) or os.path.isdir(os.path.join(ds, tname))
+
+
+def _read_scalar(path):
+    txt = open(path).read()
+    body = txt.split("internalField   nonuniform List<scalar>")[1].split("(", 1)[1].split(")", 1)[0]
+    return np.array(body.split(), float)
+
+
+def test_solve_driver_post_processing(tmp_path):
+    """lfm/post haveAverage + haveForces through the driver: pAvg / pRMS in
+    the time directories and output/postProcess_Forces_<k> per wall patch,
+    against the oracle stepping the same case."""
+    import oracle as O
+    c = sc.channel(mu=1e-3, shear=40.0)
+    dt = c.params["dt"]
+    d = str(tmp_path / "channel")
+    foam.write_case(d, c, end_steps=6, write_interval=3,
+                    post=dict(haveAverage=True, tStartAverage=2.0 * dt, haveForces=True, saveForcesStep=2))
+    r = solve.run_case(d, quiet=True)
+    assert r["steps"] == 6
+    o = O.Oracle(c.mesh, c.params, c.p, c.T, c.U, c.alpha)
+    o.set_boundary_conditions()
+    t, times, rows = 0.0, [0.0], [o.forces()]
+    for s in range(1, 7):
+        o.run(1)
+        o.average(s - 2)
+        t += o.dt
+        if s % 2 == 0:
+            times.append(t)
+            rows.append(o.forces())
+    assert np.array_equal(r["forces_time"], np.array(times))
+    scale = c.params["p_inf"] * 2 * 2.0 * 0.5
+    assert np.abs(r["forces"] - np.array(rows)).max() <= 1e-12 * scale
+    for k, name in enumerate(("bottom", "top")):
+        lines = open(os.path.join(d, "output", f"postProcess_Forces_{k + 1}")).read().splitlines()
+        assert lines[0].split()[0] == "time" and len(lines) == 1 + len(times)
+        vals = np.array([[float(x) for x in ln.split()] for ln in lines[1:]])
+        assert np.array_equal(vals[:, 0], np.array(times))
+        assert np.allclose(vals[:, 1:3], np.array(rows)[:, k, 0], rtol=1e-12, atol=1e-12 * scale)
+        assert np.allclose(vals[:, 3:5], np.array(rows)[:, k, 1], rtol=1e-12, atol=1e-12 * scale)
+        assert np.allclose(vals[:, 5:7], vals[:, 1:3] + vals[:, 3:5], rtol=1e-15, atol=0)
+    a_ref, r_ref = o.average_fields(4)
+    last = os.path.join(d, "%.12g" % r["written"][-1])  # the writer's time name
+    assert np.array_equal(_read_scalar(os.path.join(last, "pAvg")), a_ref)
+    assert np.allclose(_read_scalar(os.path.join(last, "pRMS")), r_ref, rtol=1e-13, atol=1e-13 * r_ref.max())
