@@ -1,6 +1,7 @@
 /*
  * lfm_case.h -- C ABI of the OpenFOAM case input/output around the MI355X
- * time step: the ASCII polyMesh, the dictionaries CInputReader reads and the
+ * time step: the polyMesh (ASCII or binary format), the dictionaries
+ * CInputReader reads and the
  * initial fields, as the reference takes them from OpenFOAM through
  * CPolyMeshReaderOF / CDictReaderOF / CRunTimeManagerOF
  * (polyMeshReaderOF/polyMeshReaderOF.cpp, dictReaderOF/dictReaderOF.cpp,

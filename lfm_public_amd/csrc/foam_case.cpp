@@ -10,13 +10,18 @@
 // (runTimeManagerOF/runTimeManagerOF.cpp).  This file parses the ASCII
 // formats directly: FoamFile headers, // and /* */ comments, sub-dictionaries,
 // counted lists "N ( ... )", uniform lists "N{v}", "uniform" /
-// "nonuniform List<...>" fields.  Binary polyMesh files are rejected.
+// "nonuniform List<...>" fields.  Files whose header says "format binary"
+// hold their primitive lists as raw little-endian bytes, "N(<bytes>)" and
+// "N{<bytes>}", with label and scalar sizes from the header's arch entry
+// ("LSB;label=32;scalar=64" by default); everything else in them is text.
 #include <sys/stat.h>
 
 #include <algorithm>
 #include <cctype>
 #include <cerrno>
+#include <climits>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -65,6 +70,9 @@ struct Lexer {
     const std::string &s;
     std::string file;
     size_t i = 0;
+    // binary format: primitive lists are raw bytes of these sizes
+    bool binary = false;
+    int label_bytes = 4, scalar_bytes = 8;
     Lexer(const std::string &text, const std::string &name) : s(text), file(name) {}
 
     void skip() {
@@ -137,6 +145,52 @@ struct Lexer {
         i += (size_t)(e - b);
         return v;
     }
+    // binary list payload: the opening bracket (after white space only --
+    // the bytes may contain anything), `bytes` raw bytes, the closing one
+    const char *raw(size_t bytes, char open, char close) {
+        while (i < s.size() && std::isspace((unsigned char)s[i])) i++;
+        if (i >= s.size() || s[i] != open) perr(file, std::string("expected '") + open + "' before binary data");
+        const char *p = s.data() + i + 1;
+        if (i + 1 + bytes >= s.size() || s[i + 1 + bytes] != close)
+            perr(file, "binary list of " + std::to_string(bytes) + " bytes is truncated");
+        i += bytes + 2;
+        return p;
+    }
+    // which bracket follows (white space only skipped)
+    char bracket() {
+        size_t j = i;
+        while (j < s.size() && std::isspace((unsigned char)s[j])) j++;
+        return j < s.size() ? s[j] : '\0';
+    }
+    double scalar_at(const char *p, size_t k) const {
+        if (scalar_bytes == 8) {
+            double v;
+            std::memcpy(&v, p + 8 * k, 8);
+            return v;
+        }
+        float v;
+        std::memcpy(&v, p + 4 * k, 4);
+        return v;
+    }
+    long label_at(const char *p, size_t k) const {
+        if (label_bytes == 4) {
+            int32_t v;
+            std::memcpy(&v, p + 4 * k, 4);
+            return v;
+        }
+        int64_t v;
+        std::memcpy(&v, p + 8 * k, 8);
+        return (long)v;
+    }
+    // components of one element of a List<type>
+    static int ncomp_of(const std::string &type) {
+        if (type == "scalar" || type == "label") return 1;
+        if (type == "vector") return 3;
+        if (type == "symmTensor") return 6;
+        if (type == "tensor") return 9;
+        if (type == "sphericalTensor") return 1;
+        return -1;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -190,6 +244,18 @@ void parse_dict(Lexer &L, Dict &d, bool nested) {
             int depth = 0;
             for (;;) {
                 std::string t = L.next();
+                if (L.binary && t.rfind("List<", 0) == 0 && t.back() == '>') {
+                    // "List<type> N(<bytes>)": step over the payload
+                    const int nc = Lexer::ncomp_of(t.substr(5, t.size() - 6));
+                    if (nc < 0) perr(L.file, "unsupported binary " + t);
+                    e.tokens.push_back(t);
+                    const long n = L.lab();
+                    const size_t el = (size_t)nc * (t == "List<label>" ? L.label_bytes : L.scalar_bytes);
+                    const char br = L.bracket();
+                    L.raw(br == '{' ? el : (size_t)n * el, br == '{' ? '{' : '(', br == '{' ? '}' : ')');
+                    e.tokens.push_back("<binary>");
+                    continue;
+                }
                 if (t == "(" || t == "[" || t == "{") depth++;
                 if (t == ")" || t == "]" || t == "}") depth--;
                 if (depth < 0) perr(L.file, "unbalanced bracket in entry " + key);
@@ -219,6 +285,23 @@ std::string header_word(const Dict &h, const char *k) {
     std::string t = e->tokens[0];
     if (t.size() >= 2 && t.front() == '"') t = t.substr(1, t.size() - 2);
     return t;
+}
+
+// format and arch of a FoamFile header into the lexer
+void set_format(Lexer &L, const Dict &h) {
+    L.binary = header_word(h, "format") == "binary";
+    if (!L.binary) return;
+    const std::string arch = header_word(h, "arch");
+    if (arch.find("MSB") != std::string::npos) perr(L.file, "big-endian binary data is not supported");
+    auto size_of = [&](const char *key, int dflt) {
+        const size_t p = arch.find(key);
+        if (p == std::string::npos) return dflt;
+        const int bits = std::atoi(arch.c_str() + p + std::strlen(key));
+        if (bits != 32 && bits != 64) perr(L.file, std::string("unsupported ") + key + " size in arch");
+        return bits / 8;
+    };
+    L.label_bytes = size_of("label=", 4);
+    L.scalar_bytes = size_of("scalar=", 8);
 }
 
 struct DictFile {
@@ -312,14 +395,26 @@ void open_list(const std::string &path, ListFile &lf) {
     if (!read_text(path, lf.text)) perr(path, "cannot read");
     lf.L = std::make_unique<Lexer>(lf.text, path);
     skip_header(*lf.L, &lf.header);
-    if (header_word(lf.header, "format") == "binary") perr(path, "binary format is not supported (write ascii)");
+    set_format(*lf.L, lf.header);
 }
 
-// "N ( a b ... )" or "N{v}" of labels
+int32_t to_label32(Lexer &L, long x) {
+    if (x < INT32_MIN || x > INT32_MAX) perr(L.file, "label out of the 32-bit range");
+    return (int32_t)x;
+}
+
+// "N ( a b ... )" or "N{v}" of labels (raw bytes in a binary file)
 std::vector<int32_t> read_labels(Lexer &L) {
     long n = L.lab();
     std::vector<int32_t> v;
     if (n < 0) perr(L.file, "negative list size");
+    if (L.binary) {
+        const bool uni = L.bracket() == '{';
+        const char *p = uni ? L.raw((size_t)L.label_bytes, '{', '}') : L.raw((size_t)n * L.label_bytes, '(', ')');
+        v.resize((size_t)n);
+        for (long k = 0; k < n; k++) v[k] = to_label32(L, L.label_at(p, uni ? 0 : (size_t)k));
+        return v;
+    }
     if (L.peek() == '{') {
         L.next();
         long x = L.lab();
@@ -347,7 +442,56 @@ struct Field {
     std::vector<double> v;          // n * ncomp
     std::string boundary_text;      // the boundaryField block as written
     std::string dimensions;
+    bool binary = false;            // boundary_text may hold raw lists
+    int label_bytes = 4, scalar_bytes = 8;
 };
+
+// The boundaryField block with its binary lists ("List<T> N(<bytes>)")
+// spelled out in ASCII, for the ASCII time-directory writer.
+std::string boundary_ascii(const Field &F) {
+    if (!F.binary) return F.boundary_text;
+    const std::string &t = F.boundary_text;
+    Lexer L(t, "boundaryField");
+    L.binary = true;
+    L.label_bytes = F.label_bytes;
+    L.scalar_bytes = F.scalar_bytes;
+    std::string out;
+    size_t copied = 0;
+    for (;;) {
+        const size_t at = t.find("List<", L.i);
+        if (at == std::string::npos) break;
+        const size_t close = t.find('>', at);
+        if (close == std::string::npos) break;
+        const std::string type = t.substr(at + 5, close - at - 5);
+        const int nc = Lexer::ncomp_of(type);
+        if (nc < 0) perr("boundaryField", "unsupported binary List<" + type + ">");
+        L.i = close + 1;
+        const long n = L.lab();
+        const bool lab = type == "label";
+        const size_t el = (size_t)nc * (lab ? L.label_bytes : L.scalar_bytes);
+        const bool uni = L.bracket() == '{';
+        const char *p = uni ? L.raw(el, '{', '}') : L.raw((size_t)n * el, '(', ')');
+        out += t.substr(copied, close + 1 - copied);
+        char buf[64];
+        out += " " + std::to_string(n) + (uni ? "{" : "\n(\n");
+        const long m = uni ? 1 : n;
+        for (long c = 0; c < m; c++) {
+            if (nc > 1) out += "(";
+            for (int k = 0; k < nc; k++) {
+                if (lab)
+                    std::snprintf(buf, sizeof buf, "%ld", L.label_at(p, (size_t)c * nc + k));
+                else
+                    std::snprintf(buf, sizeof buf, "%.17g", L.scalar_at(p, (size_t)c * nc + k));
+                out += (k ? " " : "") + std::string(buf);
+            }
+            out += nc > 1 ? ")" : "";
+            out += uni ? "" : "\n";
+        }
+        out += uni ? "}" : ")";
+        copied = L.i;
+    }
+    return out + t.substr(copied);
+}
 
 Field read_field(const std::string &path, long n, int ncomp) {
     Field F;
@@ -356,7 +500,10 @@ Field read_field(const std::string &path, long n, int ncomp) {
     Lexer L(text, path);
     Dict header;
     skip_header(L, &header);
-    if (header_word(header, "format") == "binary") perr(path, "binary format is not supported (write ascii)");
+    set_format(L, header);
+    F.binary = L.binary;
+    F.label_bytes = L.label_bytes;
+    F.scalar_bytes = L.scalar_bytes;
     bool got = false;
     while (!L.eof()) {
         size_t start = L.i;
@@ -379,7 +526,14 @@ Field read_field(const std::string &path, long n, int ncomp) {
                 L.next();  // List<scalar> / List<vector>
                 long m = L.lab();
                 if (m != n) perr(path, "internalField has " + std::to_string(m) + " values for " + std::to_string(n) + " cells");
-                if (L.peek() == '{') {
+                if (L.binary) {
+                    const bool uni = L.bracket() == '{';
+                    const size_t el = (size_t)ncomp * L.scalar_bytes;
+                    const char *p = uni ? L.raw(el, '{', '}') : L.raw((size_t)n * el, '(', ')');
+                    for (long c = 0; c < n; c++)
+                        for (int k = 0; k < ncomp; k++)
+                            F.v[(size_t)c * ncomp + k] = L.scalar_at(p, (uni ? 0 : (size_t)c * ncomp) + k);
+                } else if (L.peek() == '{') {
                     L.next();
                     double u[3] = {0, 0, 0};
                     if (ncomp == 1) {
@@ -489,20 +643,26 @@ static void read_mesh(lfm_case &c) {
         open_list(pm + "points", lf);
         Lexer &L = *lf.L;
         long n = L.lab();
-        L.expect('(');
         c.points.resize((size_t)n * 3);
-        for (long i = 0; i < n; i++) {
+        if (L.binary) {
+            const char *p = L.raw((size_t)n * 3 * L.scalar_bytes, '(', ')');
+            for (size_t k = 0; k < (size_t)n * 3; k++) c.points[k] = L.scalar_at(p, k);
+        } else {
             L.expect('(');
-            for (int k = 0; k < 3; k++) c.points[(size_t)i * 3 + k] = L.num();
+            for (long i = 0; i < n; i++) {
+                L.expect('(');
+                for (int k = 0; k < 3; k++) c.points[(size_t)i * 3 + k] = L.num();
+                L.expect(')');
+            }
             L.expect(')');
         }
-        L.expect(')');
     }
     {
         ListFile lf;
         open_list(pm + "faces", lf);
         Lexer &L = *lf.L;
-        if (header_word(lf.header, "class") == "faceCompactList") {
+        // binary faces are always written compact (offsets, then labels)
+        if (header_word(lf.header, "class") == "faceCompactList" || L.binary) {
             std::vector<int32_t> off = read_labels(L);
             std::vector<int32_t> pts = read_labels(L);
             c.face_off = off;
@@ -847,9 +1007,9 @@ int lfm_case_write_time(lfm_case *c, double time, const double *q, int32_t dim) 
     const std::string d = c->dir + "/" + time_name(time);
     if (!is_dir(d) && mkdir(d.c_str(), 0755) != 0) return fail(LFM_EINVAL, d + ": " + std::strerror(errno));
     try {
-        write_field(d + "/p", "volScalarField", "p", c->fp.dimensions, p, 1, c->fp.boundary_text);
-        write_field(d + "/T", "volScalarField", "T", c->fT.dimensions, T, 1, c->fT.boundary_text);
-        write_field(d + "/U", "volVectorField", "U", c->fU.dimensions, U, 3, c->fU.boundary_text);
+        write_field(d + "/p", "volScalarField", "p", c->fp.dimensions, p, 1, boundary_ascii(c->fp));
+        write_field(d + "/T", "volScalarField", "T", c->fT.dimensions, T, 1, boundary_ascii(c->fT));
+        write_field(d + "/U", "volVectorField", "U", c->fU.dimensions, U, 3, boundary_ascii(c->fU));
         const std::string bnd = calculated_boundary(*c);
         write_field(d + "/rho", "volScalarField", "rho", "[1 -3 0 0 0 0 0]", rho, 1, bnd);
         write_field(d + "/E", "volScalarField", "E", "[0 2 -2 0 0 0 0]", E, 1, bnd);

@@ -32,30 +32,58 @@ def _write(path: str, text: str):
         f.write(text)
 
 
-def write_polymesh(d: str, mesh: meshgen.PolyMesh):
-    """constant/polyMesh/{points,faces,owner,neighbour,boundary} (ASCII)."""
+_BIN_HEADER = _HEADER.replace("format      ascii;", "format      binary;\n    arch        \"LSB;label=32;scalar=64\";")
+
+
+def _bin_list(a: np.ndarray) -> bytes:
+    """OpenFOAM binary list payload: N(<raw little-endian bytes>)."""
+    return f"{a.shape[0]}\n(".encode() + np.ascontiguousarray(a).astype(a.dtype.newbyteorder("<")).tobytes() + b")\n"
+
+
+def _write_polymesh_binary(pm: str, mesh: meshgen.PolyMesh):
+    loc = "constant/polyMesh"
+    with open(os.path.join(pm, "points"), "wb") as f:
+        f.write(_BIN_HEADER.format(cls="vectorField", loc=loc, obj="points").encode())
+        pts = np.ascontiguousarray(mesh.points, np.float64)
+        f.write(f"{pts.shape[0]}\n(".encode() + pts.astype("<f8").tobytes() + b")\n")
+    with open(os.path.join(pm, "faces"), "wb") as f:
+        f.write(_BIN_HEADER.format(cls="faceCompactList", loc=loc, obj="faces").encode())
+        f.write(_bin_list(np.asarray(mesh.face_off, np.int32)))
+        f.write(_bin_list(np.asarray(mesh.face_pts, np.int32)))
+    for name, arr in (("owner", mesh.owner), ("neighbour", mesh.neighbour)):
+        with open(os.path.join(pm, name), "wb") as f:
+            f.write(_BIN_HEADER.format(cls="labelList", loc=loc, obj=name).encode())
+            f.write(_bin_list(np.asarray(arr, np.int32)))
+
+
+def write_polymesh(d: str, mesh: meshgen.PolyMesh, binary: bool = False):
+    """constant/polyMesh/{points,faces,owner,neighbour,boundary} (ASCII, or
+    OpenFOAM's binary format for the point, face and label lists)."""
     pm = os.path.join(d, "constant", "polyMesh")
     os.makedirs(pm, exist_ok=True)
     loc = "constant/polyMesh"
-    with open(os.path.join(pm, "points"), "w") as f:
-        f.write(_HEADER.format(cls="vectorField", loc=loc, obj="points"))
-        f.write(f"{mesh.points.shape[0]}\n(\n")
-        f.write("".join(f"({_r(x)} {_r(y)} {_r(z)})\n" for x, y, z in mesh.points))
-        f.write(")\n")
-    with open(os.path.join(pm, "faces"), "w") as f:
-        f.write(_HEADER.format(cls="faceList", loc=loc, obj="faces"))
-        f.write(f"{mesh.n_faces}\n(\n")
-        fo, fp = mesh.face_off, mesh.face_pts
-        f.write("".join(f"{fo[i + 1] - fo[i]}({' '.join(map(str, fp[fo[i]:fo[i + 1]]))})\n"
-                        for i in range(mesh.n_faces)))
-        f.write(")\n")
-    note = f"nPoints: {mesh.points.shape[0]} nCells: {mesh.n_cells} nFaces: {mesh.n_faces} " \
-           f"nInternalFaces: {mesh.n_internal}"
-    for name, arr in (("owner", mesh.owner), ("neighbour", mesh.neighbour)):
-        with open(os.path.join(pm, name), "w") as f:
-            f.write(_HEADER.format(cls="labelList", loc=loc, obj=name).replace(
-                "    object", f"    note        \"{note}\";\n    object"))
-            f.write(f"{arr.shape[0]}\n(\n" + "\n".join(map(str, arr.tolist())) + "\n)\n")
+    if binary:
+        _write_polymesh_binary(pm, mesh)
+    else:
+        with open(os.path.join(pm, "points"), "w") as f:
+            f.write(_HEADER.format(cls="vectorField", loc=loc, obj="points"))
+            f.write(f"{mesh.points.shape[0]}\n(\n")
+            f.write("".join(f"({_r(x)} {_r(y)} {_r(z)})\n" for x, y, z in mesh.points))
+            f.write(")\n")
+        with open(os.path.join(pm, "faces"), "w") as f:
+            f.write(_HEADER.format(cls="faceList", loc=loc, obj="faces"))
+            f.write(f"{mesh.n_faces}\n(\n")
+            fo, fp = mesh.face_off, mesh.face_pts
+            f.write("".join(f"{fo[i + 1] - fo[i]}({' '.join(map(str, fp[fo[i]:fo[i + 1]]))})\n"
+                            for i in range(mesh.n_faces)))
+            f.write(")\n")
+        note = f"nPoints: {mesh.points.shape[0]} nCells: {mesh.n_cells} nFaces: {mesh.n_faces} " \
+               f"nInternalFaces: {mesh.n_internal}"
+        for name, arr in (("owner", mesh.owner), ("neighbour", mesh.neighbour)):
+            with open(os.path.join(pm, name), "w") as f:
+                f.write(_HEADER.format(cls="labelList", loc=loc, obj=name).replace(
+                    "    object", f"    note        \"{note}\";\n    object"))
+                f.write(f"{arr.shape[0]}\n(\n" + "\n".join(map(str, arr.tolist())) + "\n)\n")
     with open(os.path.join(pm, "boundary"), "w") as f:
         f.write(_HEADER.format(cls="polyBoundaryMesh", loc=loc, obj="boundary"))
         f.write(f"{len(mesh.patches)}\n(\n")
@@ -69,9 +97,15 @@ def write_polymesh(d: str, mesh: meshgen.PolyMesh):
             f.write(f"        nFaces          {p.size};\n        startFace       {p.start};\n    }}\n")
         f.write(")\n")
     if mesh.face_proc is not None:
-        with open(os.path.join(pm, "faceProcAddressing"), "w") as f:
-            f.write(_HEADER.format(cls="labelList", loc=loc, obj="faceProcAddressing"))
-            f.write(f"{mesh.face_proc.shape[0]}\n(\n" + "\n".join(map(str, mesh.face_proc.tolist())) + "\n)\n")
+        if binary:
+            with open(os.path.join(pm, "faceProcAddressing"), "wb") as f:
+                f.write(_BIN_HEADER.format(cls="labelList", loc=loc, obj="faceProcAddressing").encode())
+                f.write(_bin_list(np.asarray(mesh.face_proc, np.int32)))
+        else:
+            with open(os.path.join(pm, "faceProcAddressing"), "w") as f:
+                f.write(_HEADER.format(cls="labelList", loc=loc, obj="faceProcAddressing"))
+                f.write(f"{mesh.face_proc.shape[0]}\n(\n" + "\n".join(map(str, mesh.face_proc.tolist())) +
+                        "\n)\n")
 
 
 def _boundary_block(mesh: meshgen.PolyMesh) -> str:
@@ -82,10 +116,27 @@ def _boundary_block(mesh: meshgen.PolyMesh) -> str:
     return out + "}\n"
 
 
-def write_fields(d: str, time_name: str, mesh: meshgen.PolyMesh, p, T, U, alpha=None):
+def _write_fields_binary(t: str, time_name: str, bnd: str, p, T, U, alpha=None):
+    def field(name, cls, dims, v, kind):
+        with open(os.path.join(t, name), "wb") as f:
+            f.write(_BIN_HEADER.format(cls=cls, loc=time_name, obj=name).encode())
+            f.write(f"dimensions      {dims};\n\ninternalField   nonuniform List<{kind}> ".encode())
+            a = np.ascontiguousarray(v, np.float64)
+            f.write(f"{a.shape[0]}\n(".encode() + a.astype("<f8").tobytes() + b")\n;\n\n" + bnd.encode())
+    field("p", "volScalarField", "[1 -1 -2 0 0 0 0]", p, "scalar")
+    field("T", "volScalarField", "[0 0 0 1 0 0 0]", T, "scalar")
+    if alpha is not None:
+        field("alpha", "volScalarField", "[0 1 0 0 0 0 0]", alpha, "scalar")
+    field("U", "volVectorField", "[0 1 -1 0 0 0 0]", np.asarray(U, np.float64).reshape(-1, 3), "vector")
+
+
+def write_fields(d: str, time_name: str, mesh: meshgen.PolyMesh, p, T, U, alpha=None, binary: bool = False):
     t = os.path.join(d, time_name)
     os.makedirs(t, exist_ok=True)
     bnd = _boundary_block(mesh)
+    if binary:
+        _write_fields_binary(t, time_name, bnd, p, T, U, alpha)
+        return
 
     def scalar(name, dims, v):
         _write(os.path.join(t, name), _HEADER.format(cls="volScalarField", loc=time_name, obj=name) +
@@ -144,20 +195,22 @@ def write_dicts(d: str, params: dict, end_time: float, write_interval: int = 100
 
 
 def write_case(d: str, case: Case, end_steps: int = 100, write_interval: int = 100,
-               cell_rank: Optional[np.ndarray] = None, post: Optional[dict] = None):
+               cell_rank: Optional[np.ndarray] = None, post: Optional[dict] = None, binary: bool = False):
     """The case as an OpenFOAM directory; with cell_rank, decomposed into
     processor<N>/ (polyMesh with processor patches and faceProcAddressing,
-    0/ fields) beside the root's system/ and constant/ dictionaries."""
+    0/ fields) beside the root's system/ and constant/ dictionaries.
+    ``binary`` writes the mesh lists and the fields in OpenFOAM's binary
+    format."""
     write_dicts(d, case.params, end_steps * case.params["dt"], write_interval, post=post)
     if cell_rank is None:
-        write_polymesh(d, case.mesh)
-        write_fields(d, "0", case.mesh, case.p, case.T, case.U, case.alpha)
+        write_polymesh(d, case.mesh, binary)
+        write_fields(d, "0", case.mesh, case.p, case.T, case.U, case.alpha, binary)
         return
     for lm, cells in meshgen.decompose(case.mesh, cell_rank):
         pd = os.path.join(d, f"processor{lm.rank}")
-        write_polymesh(pd, lm)
+        write_polymesh(pd, lm, binary)
         write_fields(pd, "0", lm, case.p[cells], case.T[cells], case.U[cells],
-                     None if case.alpha is None else case.alpha[cells])
+                     None if case.alpha is None else case.alpha[cells], binary)
 
 
 # ---------------------------------------------------------------------------

@@ -177,13 +177,81 @@ def test_write_time_round_trip(tmp_path):
     fr.close()
 
 
-def test_binary_mesh_is_rejected(tmp_path):
+@pytest.mark.parametrize("name", ["cylinder_M2AUSM_sponge", "ogrid_M1_cyclic_rk2"])
+def test_binary_case_reads_like_ascii(tmp_path, name):
+    """A case written in OpenFOAM's binary format (raw point, face, label
+    and field lists) reads back bit-identical to the ASCII one, serial and
+    decomposed, and the oracle on it steps identically."""
+    c = sc.PARITY_CASES[name]()
+    ranks = meshgen.slab_ranks(c.mesh, 2)
+    for rank_arg, sub in ((None, "serial"), (ranks, "decomposed")):
+        da, db = str(tmp_path / (sub + "_a")), str(tmp_path / (sub + "_b"))
+        foam.write_case(da, c, end_steps=5, cell_rank=rank_arg)
+        foam.write_case(db, c, end_steps=5, cell_rank=rank_arg, binary=True)
+        assert b"format      binary" in open(os.path.join(db, "processor0" if rank_arg is not None else "",
+                                                         "constant", "polyMesh", "points"), "rb").read(400)
+        for r in (foam.ranks_of(da) or [-1]):
+            fa, fb = foam.FoamCase(da, r), foam.FoamCase(db, r)
+            for k, x in fa.mesh_arrays.items():
+                if isinstance(x, np.ndarray):
+                    assert np.array_equal(x, fb.mesh_arrays[k]), k
+                else:
+                    assert x == fb.mesh_arrays[k], k
+            for k in ("p", "T", "U", "alpha"):
+                x, y = getattr(fa, k), getattr(fb, k)
+                assert (x is None and y is None) or np.array_equal(x, y), k
+
+
+def _binary_list(a):
+    return f"{a.shape[0]}\n(".encode() + a.tobytes() + b")"
+
+
+def test_binary_boundary_values_written_as_ascii(tmp_path):
+    """Binary fields with nonuniform boundary values (List<scalar>,
+    List<vector>) keep those values when a time directory is written (in
+    ASCII)."""
+    c = sc.channel()
+    d = str(tmp_path / "bb")
+    foam.write_case(d, c, end_steps=5, binary=True)
+    bottom = next(p for p in c.mesh.patches if p.name == "bottom")
+    vals = np.linspace(1.0, 2.0, bottom.size) / 7.0
+    vecs = np.arange(3 * bottom.size, dtype=np.float64).reshape(-1, 3) / 9.0
+    for name, payload, kind in (("p", vals, "scalar"), ("U", vecs, "vector")):
+        path = os.path.join(d, "0", name)
+        raw = open(path, "rb").read()
+        old = b"    bottom\n    {\n        type            zeroGradient;\n    }\n"
+        assert old in raw
+        new = (b"    bottom\n    {\n        type            fixedValue;\n        value           nonuniform List<" +
+               kind.encode() + b"> " + _binary_list(payload.astype("<f8")) + b";\n    }\n")
+        open(path, "wb").write(raw.replace(old, new))
+    fc = foam.FoamCase(d)
+    assert np.array_equal(fc.p, c.p)
+    q = np.zeros((4, c.mesh.n_cells))
+    q[0] = 1.0
+    q[3] = 2.5e5
+    fc.write_time(0.5, q, 2)
+    txt = open(os.path.join(d, "0.5", "p")).read()
+    blk = txt.split("bottom")[1].split("}")[0]
+    body = blk.split("List<scalar>")[1].split("(", 1)[1].split(")", 1)[0]
+    assert np.array_equal(np.array(body.split(), float), vals)
+    txt = open(os.path.join(d, "0.5", "U")).read()
+    blk = txt.split("bottom")[1].split("}")[0]
+    body = blk.split("List<vector>")[1].split("(", 1)[1].rsplit(")", 1)[0]
+    got = np.array(body.replace("(", " ").replace(")", " ").split(), float).reshape(-1, 3)
+    assert np.array_equal(got, vecs)
+
+
+def test_bad_binary_is_rejected(tmp_path):
     c = sc.PARITY_CASES["vortex_M1_euler"]()
     d = str(tmp_path / "bin")
-    foam.write_case(d, c, end_steps=1)
+    foam.write_case(d, c, end_steps=1, binary=True)
     pts = os.path.join(d, "constant", "polyMesh", "points")
-    open(pts, "w").write(open(pts).read().replace("format      ascii", "format      binary"))
-    with pytest.raises(capi.LfmError, match="binary"):
+    raw = open(pts, "rb").read()
+    open(pts, "wb").write(raw[:-40] + b")\n")  # truncated payload
+    with pytest.raises(capi.LfmError, match="truncated|binary"):
+        foam.FoamCase(d)
+    open(pts, "wb").write(raw.replace(b"LSB;", b"MSB;"))
+    with pytest.raises(capi.LfmError, match="big-endian"):
         foam.FoamCase(d)
 
 
