@@ -89,6 +89,27 @@ int Solver::get_field(int which, double *out) {
     }
     if (fp32) return get_field32(this, which, out);
     FieldRef r = field_ref64(this, which);
+    if (!r.ptr && (which == LFM_FIELD_TAUMC || which == LFM_FIELD_SIGMAU)) {
+        // not stored: formed from dudx and the state, as calc_VIS leaves them
+        double *dev = nullptr;
+        const size_t n = (size_t)(D * D + D) * nc;
+        HIP_TRY(hipMalloc((void **)&dev, n * sizeof(double)));
+        const int nb = (int)((nc + 255) / 256);
+        if (D == 2)
+            hipLaunchKernelGGL(vis_tensor_field<2>, dim3(nb), dim3(256), 0, stream, k64, dev, dev + (size_t)D * D * nc);
+        else
+            hipLaunchKernelGGL(vis_tensor_field<3>, dim3(nb), dim3(256), 0, stream, k64, dev, dev + (size_t)D * D * nc);
+        std::vector<double> tmp(n);
+        hipError_t e = hipMemcpyAsync(tmp.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        (void)hipFree(dev);
+        HIP_TRY(e);
+        const int ncomp = which == LFM_FIELD_TAUMC ? D * D : D;
+        const double *src = tmp.data() + (which == LFM_FIELD_TAUMC ? 0 : (size_t)D * D * nc);
+        for (int k = 0; k < ncomp; k++)
+            for (size_t ci = 0; ci < nc; ci++) out[k * nc + L.orig[ci]] = src[k * nc + ci];
+        return LFM_OK;
+    }
     if (!r.ptr) return fail(LFM_EINVAL, "unknown or unallocated field");
     std::vector<double> tmp(nc);
     for (int k = 0; k < r.ncomp; k++) {

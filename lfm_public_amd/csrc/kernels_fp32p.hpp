@@ -192,6 +192,59 @@ __device__ __forceinline__ void bc_fill32_body(const KArgs32 &a, const int g) {
 // ---------------------------------------------------------------------------
 // Green-Gauss of deviations + viscous tensors, per cell
 // ---------------------------------------------------------------------------
+// tauMC and sigmaU of one cell from its completed dudx and velocity (the
+// calc_VIS refresh, Smagorinsky stress when smag is given).  Used where they
+// are stored and where a face recomputes them.
+template <int D>
+__device__ __forceinline__ void vis_tensors32(const KArgs32 &a, const float (&dudx)[D][D], const float (&u)[D],
+                                              const float *smag_p, float (&tmc)[D][D], float (&sU)[D]) {
+    const float mu = a.mu;
+    float tau[D][D];
+#pragma unroll
+    for (int p = 0; p < D; p++) {
+        tau[p][p] = 2.0f * dudx[p][p];
+#pragma unroll
+        for (int b1 = p + 1; b1 < D + p; b1++) {
+            const int b = b1 % D;
+            tau[p][b] = mu * (dudx[p][b] + dudx[b][p]);
+            tau[p][p] -= dudx[b][b];
+        }
+        tau[p][p] *= 2.0f / 3.0f * mu;
+    }
+    float diag = 0.f;
+#pragma unroll
+    for (int p = 0; p < D; p++) diag -= dudx[p][p];
+    diag *= mu * 2.0f / 3.0f;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+#pragma unroll
+        for (int j = 0; j < D; j++) tmc[i][j] = mu * dudx[j][i];
+        tmc[i][i] += diag;
+    }
+    if (smag_p) {  // Smagorinsky stress (cfd_v0.cpp:1668-1684)
+        const float smag = *smag_p;
+        float strain = 0.f, divu = 0.f;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                const float sij = 0.5f * (dudx[i][j] + dudx[j][i]);
+                strain += sij * sij;
+            }
+            divu += dudx[i][i];
+        }
+        strain = sqrt32(2.f * strain);
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) tmc[i][j] -= smag * strain * (0.5f * (dudx[i][j] + dudx[j][i]));
+            tmc[i][i] += smag * strain * divu * (1.f / 3.f);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) sU[i] = dot32<D>(u, tau[i]);
+}
+
 template <int D, int GRAD, bool VISC>
 __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
     if (c >= a.nc) return;
@@ -246,60 +299,25 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
             }
         }
     }
-    const Prim32<D> &sc = own;
-    const float mu = a.mu;
-    float tau[D][D];
-#pragma unroll
-    for (int p = 0; p < D; p++) {
-        tau[p][p] = 2.0f * dudx[p][p];
-#pragma unroll
-        for (int b1 = p + 1; b1 < D + p; b1++) {
-            const int b = b1 % D;
-            tau[p][b] = mu * (dudx[p][b] + dudx[b][p]);
-            tau[p][p] -= dudx[b][b];
-        }
-        tau[p][p] *= 2.0f / 3.0f * mu;
-    }
-    float diag = 0.f;
-#pragma unroll
-    for (int p = 0; p < D; p++) diag -= dudx[p][p];
-    diag *= mu * 2.0f / 3.0f;
+    // tauMC / sigmaU stored only where a halo payload needs them (otherwise
+    // each face recomputes them from dudx); U_grad equals dudx bit for bit
+    // (same face velocity, same sums) and shares its storage unless a halo
+    // payload must leave it zero on the ghosts
     const size_t nt = a.ntot;
-    float tmc[D][D];
-#pragma unroll
-    for (int i = 0; i < D; i++) {
-#pragma unroll
-        for (int j = 0; j < D; j++) tmc[i][j] = mu * dudx[j][i];
-        tmc[i][i] += diag;
-    }
-    if (VISC && a.smag) {  // Smagorinsky stress (cfd_v0.cpp:1668-1684)
-        const float smag = a.smag[c];
-        float strain = 0.f, divu = 0.f;
+    if (VISC && a.tauMC) {
+        float tmc[D][D], sU[D];
+        vis_tensors32<D>(a, dudx, own.u, a.smag ? a.smag + c : nullptr, tmc, sU);
 #pragma unroll
         for (int i = 0; i < D; i++) {
+            a.sigmaU[i * nt + c] = sU[i];
 #pragma unroll
-            for (int j = 0; j < D; j++) {
-                const float sij = 0.5f * (dudx[i][j] + dudx[j][i]);
-                strain += sij * sij;
-            }
-            divu += dudx[i][i];
-        }
-        strain = sqrt32(2.f * strain);
-#pragma unroll
-        for (int i = 0; i < D; i++) {
-#pragma unroll
-            for (int j = 0; j < D; j++) tmc[i][j] -= smag * strain * (0.5f * (dudx[i][j] + dudx[j][i]));
-            tmc[i][i] += smag * strain * divu * (1.f / 3.f);
+            for (int j = 0; j < D; j++) a.tauMC[(i * D + j) * nt + c] = tmc[i][j];
         }
     }
 #pragma unroll
     for (int i = 0; i < D; i++) {
-        if (VISC) a.sigmaU[i * nt + c] = dot32<D>(sc.u, tau[i]);
 #pragma unroll
-        for (int j = 0; j < D; j++) {
-            if (VISC) a.tauMC[(i * D + j) * nt + c] = tmc[i][j];
-            a.dudx[(i * D + j) * nt + c] = dudx[i][j];
-        }
+        for (int j = 0; j < D; j++) a.dudx[(i * D + j) * nt + c] = dudx[i][j];
         if (VISC) a.dTdx[i * nt + c] = dTdx[i];
     }
     if (GRAD == 2) {
@@ -307,16 +325,19 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
         for (int i = 0; i < D; i++) {
             a.rho_grad[i * nt + c] = rg[i];
             a.p_grad[i * nt + c] = pg[i];
-#pragma unroll
-            for (int j = 0; j < D; j++) a.U_grad[(i * D + j) * nt + c] = ug[i][j];
         }
+        if (a.U_grad != a.dudx)
+#pragma unroll
+            for (int i = 0; i < D; i++)
+#pragma unroll
+                for (int j = 0; j < D; j++) a.U_grad[(i * D + j) * nt + c] = ug[i][j];
     }
 }
 
 // ---------------------------------------------------------------------------
 // face fluxes minus the freestream flux, one thread per face
 // ---------------------------------------------------------------------------
-template <int D, int SOLVER, bool VISC>
+template <int D, int SOLVER, bool VISC, bool STORED>
 __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) {
     if (f >= a.nf) return;
     const int P = a.fP[f], N = a.fN[f];
@@ -489,6 +510,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
     for (int k = 0; k < D; k++) dmag2 += d[k] * d[k];
     const float dmi = rcp32(sqrt32(dmag2));
     float dn[D], du[D][D], dT[D], tau[D][D], tmc[D][D];
+    float duP[D][D], duN[D][D], sUP[D], sUN[D];
 #pragma unroll
     for (int k = 0; k < D; k++) dn[k] = d[k] * dmi;
     const float cT = sp.Rp * a.R_inv, aT = sn.Rp * a.R_inv;
@@ -503,8 +525,11 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
 #pragma unroll
         for (int i = 0; i < D; i++) {
 #pragma unroll
-            for (int j = 0; j < D; j++)
-                du[i][j] = LERP32(w, a.dudx[(i * D + j) * nt + P], a.dudx[(i * D + j) * nt + N]);
+            for (int j = 0; j < D; j++) {
+                duP[i][j] = a.dudx[(i * D + j) * nt + P];
+                duN[i][j] = a.dudx[(i * D + j) * nt + N];
+                du[i][j] = LERP32(w, duP[i][j], duN[i][j]);
+            }
             dT[i] = LERP32(w, a.dTdx[i * nt + P], a.dTdx[i * nt + N]);
         }
     }
@@ -530,12 +555,21 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
             for (int j = 0; j < D; j++) tmc[i][j] = mu * du[j][i];
             tmc[i][i] += diag;
         }
-    } else {
+    } else if constexpr (STORED) {
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
             for (int j = 0; j < D; j++)
                 tmc[i][j] = LERP32(w, a.tauMC[(i * D + j) * nt + P], a.tauMC[(i * D + j) * nt + N]);
+    } else {
+        // both cells' tensors as calc_VIS leaves them, from their dudx
+        float tP[D][D], tN[D][D];
+        vis_tensors32<D>(a, duP, sp.u, a.smag ? a.smag + P : nullptr, tP, sUP);
+        vis_tensors32<D>(a, duN, sn.u, a.smag ? a.smag + N : nullptr, tN, sUN);
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) tmc[i][j] = LERP32(w, tP[i][j], tN[i][j]);
     }
     const float Sd = dot32<D>(S, d);
     float K[D], dm2 = 0.f;
@@ -553,9 +587,12 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
         for (int i = 0; i < D; i++) Uf[i] = LERP32(w, sp.u[i], sn.u[i]);
 #pragma unroll
         for (int i = 0; i < D; i++) sU[i] = dot32<D>(Uf, tau[i]);
-    } else {
+    } else if constexpr (STORED) {
 #pragma unroll
         for (int i = 0; i < D; i++) sU[i] = LERP32(w, a.sigmaU[i * nt + P], a.sigmaU[i * nt + N]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++) sU[i] = LERP32(w, sUP[i], sUN[i]);
     }
     const float kcond = a.cp * mu * a.pr_inv;
 #pragma unroll
@@ -761,9 +798,9 @@ template <int D, int GRAD, bool VISC>
 __global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
     grad_vis32_body<D, GRAD, VISC>(a, LFM_TID(blockIdx.x));
 }
-template <int D, int SOLVER, bool VISC>
+template <int D, int SOLVER, bool VISC, bool STORED>
 __global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
-    face_flux32_body<D, SOLVER, VISC>(a, LFM_TID(blockIdx.x));
+    face_flux32_body<D, SOLVER, VISC, STORED>(a, LFM_TID(blockIdx.x));
 }
 template <int D, bool RES>
 __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B, int zero_dq) {
@@ -772,6 +809,29 @@ __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
     cfl_dt_cells32_body<D, MODE>(a, param, LFM_TID(blockIdx.x), LFM_BLK());
+}
+
+// tauMC and sigmaU of every cell from the stored dudx (field read-back of a
+// solver that does not store them): [D*D][nc] and [D][nc] deviations-free
+// tensors (they do not depend on the free stream)
+template <int D>
+__global__ void __launch_bounds__(256) vis_tensor_field32(KArgs32 a, float *tmc_out, float *sU_out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nc) return;
+    float du[D][D], q[D + 2], tmc[D][D], sU[D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) du[i][j] = a.dudx[(size_t)(i * D + j) * a.ntot + c];
+    load_q32<D>(a, c, q);
+    const Prim32<D> s = prim32<D>(a, q);
+    vis_tensors32<D>(a, du, s.u, a.smag ? a.smag + c : nullptr, tmc, sU);
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        sU_out[(size_t)i * a.nc + c] = sU[i];
+#pragma unroll
+        for (int j = 0; j < D; j++) tmc_out[(size_t)(i * D + j) * a.nc + c] = tmc[i][j];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -797,11 +857,12 @@ __global__ void __launch_bounds__(256) grad_vis32_b(const KArgs32 *args, BatchMa
     batch_slot(m, r, lb);
     grad_vis32_body<D, GRAD, VISC>(args[r], lb * 256 + threadIdx.x);
 }
+// (batch members are undecomposed: their tensors are never stored)
 template <int D, int SOLVER, bool VISC>
 __global__ void __launch_bounds__(256) face_flux32_b(const KArgs32 *args, BatchMap m) {
     int r, lb;
     batch_slot(m, r, lb);
-    face_flux32_body<D, SOLVER, VISC>(args[r], lb * 256 + threadIdx.x);
+    face_flux32_body<D, SOLVER, VISC, false>(args[r], lb * 256 + threadIdx.x);
 }
 template <int D>
 __global__ void __launch_bounds__(256) cell_update32_b(const KArgs32 *args, BatchMap m, float A, float B,

@@ -200,6 +200,67 @@ __global__ void __launch_bounds__(256) bc_fill(KArgs a) {
 // multiplied by mu, so they are not formed; dudx is kept for the M2AUSM
 // sensor.  (For solvers 0/1 with mu = 0 the kernel is not launched.)
 // ---------------------------------------------------------------------------
+// tauMC and sigmaU of one cell from its completed dudx (cfd_v0.cpp:1813-1834,
+// Smagorinsky 1668-1684 when smag is given): the values calc_VIS leaves in
+// the cell.  Used where they are stored and where a face recomputes them.
+template <int D>
+__device__ __forceinline__ void vis_tensors(const KArgs &a, const double (&dudx)[D][D], const double *qc,
+                                            const double *smag_p, double (&tmc)[D][D], double (&sU)[D]) {
+    double U[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) U[k] = qc[k + 1] / qc[0];
+    const double mu = a.mu;
+    double tau[D][D];
+#pragma unroll
+    for (int p = 0; p < D; p++) {
+        tau[p][p] = 2.0 * dudx[p][p];
+#pragma unroll
+        for (int b1 = p + 1; b1 < D + p; b1++) {
+            const int b = b1 % D;
+            tau[p][b] = mu * (dudx[p][b] + dudx[b][p]);
+            tau[p][p] -= dudx[b][b];
+        }
+        tau[p][p] *= 2.0 / 3.0 * mu;
+    }
+    double diag = 0;
+#pragma unroll
+    for (int p = 0; p < D; p++) diag -= dudx[p][p];
+    diag *= mu * 2.0 / 3.0;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+#pragma unroll
+        for (int j = 0; j < D; j++) tmc[i][j] = mu * dudx[j][i];
+        tmc[i][i] += diag;
+    }
+    if (smag_p) {
+        // calc_VIS_Smagorinsky: minus the eddy stress smag |S| S_ij, plus the
+        // compressible correction smag |S| div u / 3
+        const double smag = *smag_p;
+        double strain = 0.0, divu = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                const double sij = 0.5 * (dudx[i][j] + dudx[j][i]);
+                strain += sij * sij;
+            }
+            divu += dudx[i][i];
+        }
+        strain = sqrt(2.0 * strain);
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                const double sij = 0.5 * (dudx[i][j] + dudx[j][i]);
+                tmc[i][j] -= smag * strain * sij;
+            }
+            tmc[i][i] += smag * strain * divu / 3.0;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) sU[i] = dotD<D>(U, tau[i]);
+}
+
 template <int D, int GRAD, bool VISC>
 __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
     const int c = LFM_TID(blockIdx.x);
@@ -278,69 +339,24 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
             if (VISC) dTdx[i] += fT * sv[i];
         }
     }
-    // refresh of tau / sigmaU / tauMC from the completed dudx (cfd_v0.cpp:1813-1834)
-    const double *qc = qself;
-    double U[D];
-#pragma unroll
-    for (int k = 0; k < D; k++) U[k] = qc[k + 1] / qc[0];
-    const double mu = a.mu;
-    double tau[D][D];
-#pragma unroll
-    for (int p = 0; p < D; p++) {
-        tau[p][p] = 2.0 * dudx[p][p];
-#pragma unroll
-        for (int b1 = p + 1; b1 < D + p; b1++) {
-            const int b = b1 % D;
-            tau[p][b] = mu * (dudx[p][b] + dudx[b][p]);
-            tau[p][p] -= dudx[b][b];
-        }
-        tau[p][p] *= 2.0 / 3.0 * mu;
-    }
-    double diag = 0;
-#pragma unroll
-    for (int p = 0; p < D; p++) diag -= dudx[p][p];
-    diag *= mu * 2.0 / 3.0;
+    // refresh of tau / sigmaU / tauMC from the completed dudx (cfd_v0.cpp:1813-1834);
+    // stored only where a halo payload needs them (otherwise each face
+    // recomputes them from dudx)
     const size_t nt = a.ntot;
-    double tmc[D][D];
-#pragma unroll
-    for (int i = 0; i < D; i++) {
-#pragma unroll
-        for (int j = 0; j < D; j++) tmc[i][j] = mu * dudx[j][i];
-        tmc[i][i] += diag;
-    }
-    if (VISC && a.smag) {
-        // calc_VIS_Smagorinsky (cfd_v0.cpp:1668-1684): minus the eddy stress
-        // smag |S| S_ij, plus the compressible correction smag |S| div u / 3
-        const double smag = a.smag[c];
-        double strain = 0.0, divu = 0.0;
+    if (VISC && a.tauMC) {
+        double tmc[D][D], sU[D];
+        vis_tensors<D>(a, dudx, qself, a.smag ? a.smag + c : nullptr, tmc, sU);
 #pragma unroll
         for (int i = 0; i < D; i++) {
+            a.sigmaU[i * nt + c] = sU[i];
 #pragma unroll
-            for (int j = 0; j < D; j++) {
-                const double sij = 0.5 * (dudx[i][j] + dudx[j][i]);
-                strain += sij * sij;
-            }
-            divu += dudx[i][i];
-        }
-        strain = sqrt(2.0 * strain);
-#pragma unroll
-        for (int i = 0; i < D; i++) {
-#pragma unroll
-            for (int j = 0; j < D; j++) {
-                const double sij = 0.5 * (dudx[i][j] + dudx[j][i]);
-                tmc[i][j] -= smag * strain * sij;
-            }
-            tmc[i][i] += smag * strain * divu / 3.0;
+            for (int j = 0; j < D; j++) a.tauMC[(i * D + j) * nt + c] = tmc[i][j];
         }
     }
 #pragma unroll
     for (int i = 0; i < D; i++) {
-        if (VISC) a.sigmaU[i * nt + c] = dotD<D>(U, tau[i]);
 #pragma unroll
-        for (int j = 0; j < D; j++) {
-            if (VISC) a.tauMC[(i * D + j) * nt + c] = tmc[i][j];
-            a.dudx[(i * D + j) * nt + c] = dudx[i][j];
-        }
+        for (int j = 0; j < D; j++) a.dudx[(i * D + j) * nt + c] = dudx[i][j];
         if (VISC) a.dTdx[i * nt + c] = dTdx[i];
     }
     if (GRAD == 2) {
@@ -359,8 +375,10 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
 // the viscous terms shared by all three (2682-2790).  One thread per face.
 // VISC false (mu = 0): every viscous term is mu or k = cp mu / Pr times a
 // finite value, i.e. a zero the reference adds; it is not formed.
+// STORED: tauMC / sigmaU read from the arrays grad_vis filled (partitions,
+// whose halo payload carries them); otherwise formed here from dudx.
 // ---------------------------------------------------------------------------
-template <int D, int SOLVER, bool VISC>
+template <int D, int SOLVER, bool VISC, bool STORED>
 __global__ void __launch_bounds__(256) face_flux(KArgs a) {
     const int f = LFM_TID(blockIdx.x);
     if (f >= a.nf) return;
@@ -549,6 +567,7 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
     dmag = sqrt(dmag);
     const double dmi = 1.0 / dmag;
     double dn[D], du[D][D], dT[D], tau[D][D], tmc[D][D];
+    double duP[D][D], duN[D][D], sUP[D], sUN[D];
 #pragma unroll
     for (int k = 0; k < D; k++) dn[k] = d[k] * dmi;
     if (ghost) {
@@ -562,8 +581,11 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
 #pragma unroll
         for (int i = 0; i < D; i++) {
 #pragma unroll
-            for (int j = 0; j < D; j++)
-                du[i][j] = INTERP_LINEAR(w, a.dudx[(i * D + j) * nt + P], a.dudx[(i * D + j) * nt + N]);
+            for (int j = 0; j < D; j++) {
+                duP[i][j] = a.dudx[(i * D + j) * nt + P];
+                duN[i][j] = a.dudx[(i * D + j) * nt + N];
+                du[i][j] = INTERP_LINEAR(w, duP[i][j], duN[i][j]);
+            }
             dT[i] = INTERP_LINEAR(w, a.dTdx[i * nt + P], a.dTdx[i * nt + N]);
         }
     }
@@ -590,12 +612,21 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
             for (int j = 0; j < D; j++) tmc[i][j] = mu * du[j][i];
             tmc[i][i] += diag;
         }
-    } else {
+    } else if constexpr (STORED) {
 #pragma unroll
         for (int i = 0; i < D; i++)
 #pragma unroll
             for (int j = 0; j < D; j++)
                 tmc[i][j] = INTERP_LINEAR(w, a.tauMC[(i * D + j) * nt + P], a.tauMC[(i * D + j) * nt + N]);
+    } else {
+        // both cells' tensors as calc_VIS leaves them, from their dudx
+        double tP[D][D], tN[D][D];
+        vis_tensors<D>(a, duP, qP, a.smag ? a.smag + P : nullptr, tP, sUP);
+        vis_tensors<D>(a, duN, qN, a.smag ? a.smag + N : nullptr, tN, sUN);
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+            for (int j = 0; j < D; j++) tmc[i][j] = INTERP_LINEAR(w, tP[i][j], tN[i][j]);
     }
     const double Sd = dotD<D>(S, d);
     double delta[D], K[D], dm = 0.0;
@@ -613,9 +644,12 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
         for (int i = 0; i < D; i++) Uf[i] = INTERP_LINEAR(w, cU[i], aU[i]);
 #pragma unroll
         for (int i = 0; i < D; i++) sU[i] = dotD<D>(Uf, tau[i]);
-    } else {
+    } else if constexpr (STORED) {
 #pragma unroll
         for (int i = 0; i < D; i++) sU[i] = INTERP_LINEAR(w, a.sigmaU[i * nt + P], a.sigmaU[i * nt + N]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++) sU[i] = INTERP_LINEAR(w, sUP[i], sUN[i]);
     }
 #pragma unroll
     for (int i = 0; i < D; i++) {
@@ -628,6 +662,27 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < D + 2; k++) a.frhs[(size_t)k * a.nf + f] = rhs[k];
+}
+
+// tauMC and sigmaU of every cell from the stored dudx (field read-back of a
+// solver that does not store them): [D*D][nc] and [D][nc]
+template <int D>
+__global__ void __launch_bounds__(256) vis_tensor_field(KArgs a, double *tmc_out, double *sU_out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nc) return;
+    double du[D][D], q[D + 2], tmc[D][D], sU[D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int j = 0; j < D; j++) du[i][j] = a.dudx[(size_t)(i * D + j) * a.ntot + c];
+    load_q<D>(a, c, q);
+    vis_tensors<D>(a, du, q, a.smag ? a.smag + c : nullptr, tmc, sU);
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        sU_out[(size_t)i * a.nc + c] = sU[i];
+#pragma unroll
+        for (int j = 0; j < D; j++) tmc_out[(size_t)(i * D + j) * a.nc + c] = tmc[i][j];
+    }
 }
 
 // ---------------------------------------------------------------------------

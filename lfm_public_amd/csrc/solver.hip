@@ -112,8 +112,12 @@ static int create_fp64(Solver *s) {
     rc |= dev_zeros(s, &a.res_cell, (size_t)(D + 2) * nc);
     rc |= dev_zeros(s, &a.dudx, (size_t)D * D * ntot);
     rc |= dev_zeros(s, &a.dTdx, (size_t)D * ntot);
-    rc |= dev_zeros(s, &a.tauMC, (size_t)D * D * ntot);
-    rc |= dev_zeros(s, &a.sigmaU, (size_t)D * ntot);
+    // tauMC / sigmaU are kept only for the halo payload of a partition; the
+    // face kernel otherwise recomputes them from dudx
+    if (!L.halo.empty()) {
+        rc |= dev_zeros(s, &a.tauMC, (size_t)D * D * ntot);
+        rc |= dev_zeros(s, &a.sigmaU, (size_t)D * ntot);
+    }
     if (s->prm.solver == 2 && s->prm.minmod) {
         rc |= dev_zeros(s, &a.rho_grad, (size_t)D * ntot);
         rc |= dev_zeros(s, &a.p_grad, (size_t)D * ntot);
@@ -180,22 +184,22 @@ static void launch_gradvis64(Solver *s, bool full) {
         launch_gradvis64_v<D, false>(s);
 }
 
-template <int D, bool V>
+template <int D, bool V, bool ST>
 static void launch_flux64_v(Solver *s) {
     const int nb = nblocks(s->L.nf);
     switch (s->prm.solver) {
-    case 0: hipLaunchKernelGGL((face_flux<D, 0, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
-    case 1: hipLaunchKernelGGL((face_flux<D, 1, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
-    default: hipLaunchKernelGGL((face_flux<D, 2, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
+    case 0: hipLaunchKernelGGL((face_flux<D, 0, V, ST>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
+    case 1: hipLaunchKernelGGL((face_flux<D, 1, V, ST>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
+    default: hipLaunchKernelGGL((face_flux<D, 2, V, ST>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64); break;
     }
 }
 
 template <int D>
 static void launch_flux64(Solver *s) {
     if (s->visc)
-        launch_flux64_v<D, true>(s);
+        s->k64.tauMC ? launch_flux64_v<D, true, true>(s) : launch_flux64_v<D, true, false>(s);
     else
-        launch_flux64_v<D, false>(s);
+        launch_flux64_v<D, false, false>(s);
 }
 
 template <int D>

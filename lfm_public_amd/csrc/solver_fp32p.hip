@@ -128,12 +128,22 @@ int create_fp32p(Solver *s) {
     rc |= up(s, &a.res_cell, (float *)nullptr, (size_t)(D + 2) * nc);
     rc |= up(s, &a.dudx, (float *)nullptr, (size_t)D * D * ntot);
     rc |= up(s, &a.dTdx, (float *)nullptr, (size_t)D * ntot);
-    rc |= up(s, &a.tauMC, (float *)nullptr, (size_t)D * D * ntot);
-    rc |= up(s, &a.sigmaU, (float *)nullptr, (size_t)D * ntot);
+    // tauMC / sigmaU are kept only for the halo payload of a partition (the
+    // face kernel otherwise recomputes them from dudx); U_grad shares dudx's
+    // storage (equal bit for bit) except where halo ghosts must read zero
+    // U_grad, or where no gradient pass writes it (solver 2 without minmod)
+    const bool part = !L.halo.empty();
+    if (part) {
+        rc |= up(s, &a.tauMC, (float *)nullptr, (size_t)D * D * ntot);
+        rc |= up(s, &a.sigmaU, (float *)nullptr, (size_t)D * ntot);
+    }
     if (s->prm.solver == 2) {
         rc |= up(s, &a.rho_grad, (float *)nullptr, (size_t)D * ntot);
         rc |= up(s, &a.p_grad, (float *)nullptr, (size_t)D * ntot);
-        rc |= up(s, &a.U_grad, (float *)nullptr, (size_t)D * D * ntot);
+        if (part || !s->prm.minmod)
+            rc |= up(s, &a.U_grad, (float *)nullptr, (size_t)D * D * ntot);
+        else
+            a.U_grad = a.dudx;
     }
     rc |= up(s, &a.frhs, (float *)nullptr, (size_t)(D + 2) * nf);
     const size_t npart = (size_t)nblocks((long)std::max(nc, nf)) * (D + 2) + 16;
@@ -174,22 +184,22 @@ void launch_gradvis32(Solver *s, bool full) {
         launch_gradvis32_v<D, false>(s);
 }
 
-template <int D, bool V>
+template <int D, bool V, bool ST>
 static void launch_flux32_v(Solver *s) {
     const int nb = nblocks(s->L.nf);
     switch (s->prm.solver) {
-    case 0: hipLaunchKernelGGL((face_flux32<D, 0, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
-    case 1: hipLaunchKernelGGL((face_flux32<D, 1, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
-    default: hipLaunchKernelGGL((face_flux32<D, 2, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
+    case 0: hipLaunchKernelGGL((face_flux32<D, 0, V, ST>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
+    case 1: hipLaunchKernelGGL((face_flux32<D, 1, V, ST>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
+    default: hipLaunchKernelGGL((face_flux32<D, 2, V, ST>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32); break;
     }
 }
 
 template <int D>
 void launch_flux32(Solver *s) {
     if (s->visc)
-        launch_flux32_v<D, true>(s);
+        s->k32->tauMC ? launch_flux32_v<D, true, true>(s) : launch_flux32_v<D, true, false>(s);
     else
-        launch_flux32_v<D, false>(s);
+        launch_flux32_v<D, false, false>(s);
 }
 
 template <int D>
@@ -252,6 +262,28 @@ int get_field32(Solver *s, int which, double *out) {
     case LFM_FIELD_UGRAD: ptr = a.U_grad; ncomp = D * D; break;
     case LFM_FIELD_RES: ptr = a.res_cell; ncomp = D + 2; stride = nc; break;
     default: return fail(LFM_EINVAL, "unknown field");
+    }
+    std::vector<float> derived;
+    if (!ptr && (which == LFM_FIELD_TAUMC || which == LFM_FIELD_SIGMAU)) {
+        // not stored: formed from dudx and the state
+        float *dev = nullptr;
+        HIP_TRY(hipMalloc((void **)&dev, (size_t)(D * D + D) * nc * sizeof(float)));
+        if (D == 2)
+            hipLaunchKernelGGL(vis_tensor_field32<2>, dim3(nblocks((long)nc)), dim3(kBlock), 0, s->stream, a, dev,
+                               dev + (size_t)D * D * nc);
+        else
+            hipLaunchKernelGGL(vis_tensor_field32<3>, dim3(nblocks((long)nc)), dim3(kBlock), 0, s->stream, a, dev,
+                               dev + (size_t)D * D * nc);
+        derived.resize((size_t)(D * D + D) * nc);
+        hipError_t e = hipMemcpyAsync(derived.data(), dev, derived.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                      s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        (void)hipFree(dev);
+        HIP_TRY(e);
+        const float *src = derived.data() + (which == LFM_FIELD_TAUMC ? 0 : (size_t)D * D * nc);
+        for (int k = 0; k < ncomp; k++)
+            for (size_t ci = 0; ci < nc; ci++) out[k * nc + L.orig[ci]] = (double)src[k * nc + ci];
+        return LFM_OK;
     }
     if (!ptr) return fail(LFM_EINVAL, "field not allocated for this solver");
     double qinf[5] = {a.rho_inf_d, a.m_inf_d[0], a.m_inf_d[1], a.m_inf_d[2], 0.0};
