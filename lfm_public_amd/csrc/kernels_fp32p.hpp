@@ -214,7 +214,7 @@ __device__ __forceinline__ void vis_tensors32(const KArgs32 &a, const float (&du
     float diag = 0.f;
 #pragma unroll
     for (int p = 0; p < D; p++) diag -= dudx[p][p];
-    diag *= mu * 2.0f / 3.0f;
+    diag *= mu * (2.0f / 3.0f);
 #pragma unroll
     for (int i = 0; i < D; i++) {
 #pragma unroll
@@ -245,11 +245,13 @@ __device__ __forceinline__ void vis_tensors32(const KArgs32 &a, const float (&du
     for (int i = 0; i < D; i++) sU[i] = dot32<D>(u, tau[i]);
 }
 
+// Green-Gauss sums of one cell over its faces in gather order; returns the
+// cell's own primitive state
 template <int D, int GRAD, bool VISC>
-__device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
-    if (c >= a.nc) return;
+__device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, float (&dudx)[D][D],
+                                                  float (&dTdx)[D], float (&rg)[D], float (&pg)[D],
+                                                  float (&ug)[D][D]) {
     const float vinv = a.vinv[c];
-    float dudx[D][D], dTdx[D], rg[D], pg[D], ug[D][D];
 #pragma unroll
     for (int i = 0; i < D; i++) {
         dTdx[i] = rg[i] = pg[i] = 0.f;
@@ -299,6 +301,14 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
             }
         }
     }
+    return own;
+}
+
+template <int D, int GRAD, bool VISC>
+__device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
+    if (c >= a.nc) return;
+    float dudx[D][D], dTdx[D], rg[D], pg[D], ug[D][D];
+    const Prim32<D> own = grad32_sums<D, GRAD, VISC>(a, c, dudx, dTdx, rg, pg, ug);
     // tauMC / sigmaU stored only where a halo payload needs them (otherwise
     // each face recomputes them from dudx); U_grad equals dudx bit for bit
     // (same face velocity, same sums) and shares its storage unless a halo
@@ -335,11 +345,25 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
 }
 
 // ---------------------------------------------------------------------------
+// Where the face kernel reads the cells' gradients (the solver's arrays,
+// indexed by cell)
+// ---------------------------------------------------------------------------
+template <int D>
+struct GradGlobal32 {
+    const KArgs32 &a;
+    __device__ __forceinline__ float du(int c, int i, int j) const { return a.dudx[(size_t)(i * D + j) * a.ntot + c]; }
+    __device__ __forceinline__ float dT(int c, int i) const { return a.dTdx[(size_t)i * a.ntot + c]; }
+    __device__ __forceinline__ float rg(int c, int i) const { return a.rho_grad[(size_t)i * a.ntot + c]; }
+    __device__ __forceinline__ float pg(int c, int i) const { return a.p_grad[(size_t)i * a.ntot + c]; }
+    __device__ __forceinline__ float ug(int c, int i, int j) const { return a.U_grad[(size_t)(i * D + j) * a.ntot + c]; }
+};
+
+// ---------------------------------------------------------------------------
 // face fluxes minus the freestream flux, one thread per face
 // ---------------------------------------------------------------------------
-template <int D, int SOLVER, bool VISC, bool STORED>
-__device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) {
-    if (f >= a.nf) return;
+template <int D, int SOLVER, bool VISC, bool STORED, class GA>
+__device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, const GA &ga, const int gP,
+                                                const int gN, float (&rhs)[D + 2]) {
     const int P = a.fP[f], N = a.fN[f];
     // is_ghost: boundary ghosts only; halo ghosts carry the neighbour's tensors
     const bool ghost = (unsigned)(N - a.nc) < (unsigned)a.ng_bnd;
@@ -361,7 +385,6 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
     float phi_inf = 0.f;
 #pragma unroll
     for (int k = 0; k < D; k++) phi_inf += a.uinf[k] * S[k];
-    float rhs[D + 2];
 
     if (SOLVER == 0) {
         // reconstructed state (cfd_v0.cpp:2590-2637): conservatives, Rpsi
@@ -427,8 +450,8 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
             for (int i = 0; i < D; i++)
 #pragma unroll
                 for (int j = 0; j < D; j++) {
-                    cdu[i][j] = a.dudx[(i * D + j) * nt + P];
-                    ndu[i][j] = a.dudx[(i * D + j) * nt + N];
+                    cdu[i][j] = ga.du(gP, i, j);
+                    ndu[i][j] = ga.du(gN, i, j);
                 }
 #pragma unroll
             for (int p = 0; p < D; p++) {
@@ -442,8 +465,8 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
                 float gc[D], gn[D];
 #pragma unroll
                 for (int k = 0; k < D; k++) {
-                    gc[k] = a.U_grad[(p * D + k) * nt + P];
-                    gn[k] = a.U_grad[(p * D + k) * nt + N];
+                    gc[k] = ga.ug(gP, p, k);
+                    gn[k] = ga.ug(gN, p, k);
                 }
                 unP += minmod32<D>(sp.up[p], sn.up[p], gc, d, w, 1.0f) * S[p];
                 unN += minmod32<D>(sp.up[p], sn.up[p], gn, d, w, -1.0f) * S[p];
@@ -454,10 +477,10 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
             float gc[D], gn[D], hc[D], hn[D];
 #pragma unroll
             for (int k = 0; k < D; k++) {
-                gc[k] = a.rho_grad[k * nt + P];
-                gn[k] = a.rho_grad[k * nt + N];
-                hc[k] = a.p_grad[k * nt + P];
-                hn[k] = a.p_grad[k * nt + N];
+                gc[k] = ga.rg(gP, k);
+                gn[k] = ga.rg(gN, k);
+                hc[k] = ga.pg(gP, k);
+                hn[k] = ga.pg(gN, k);
             }
             const float rhoPos = a.rho_inf + minmod32<D>(sp.rp, sn.rp, gc, d, w, 1.0f);
             const float rhoNeg = a.rho_inf + minmod32<D>(sp.rp, sn.rp, gn, d, w, -1.0f);
@@ -526,11 +549,11 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
         for (int i = 0; i < D; i++) {
 #pragma unroll
             for (int j = 0; j < D; j++) {
-                duP[i][j] = a.dudx[(i * D + j) * nt + P];
-                duN[i][j] = a.dudx[(i * D + j) * nt + N];
+                duP[i][j] = ga.du(gP, i, j);
+                duN[i][j] = ga.du(gN, i, j);
                 du[i][j] = LERP32(w, duP[i][j], duN[i][j]);
             }
-            dT[i] = LERP32(w, a.dTdx[i * nt + P], a.dTdx[i * nt + N]);
+            dT[i] = LERP32(w, ga.dT(gP, i), ga.dT(gN, i));
         }
     }
 #pragma unroll
@@ -548,7 +571,7 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
         float diag = 0.f;
 #pragma unroll
         for (int p = 0; p < D; p++) diag -= du[p][p];
-        diag *= mu * 2.0f / 3.0f;
+        diag *= mu * (2.0f / 3.0f);
 #pragma unroll
         for (int i = 0; i < D; i++) {
 #pragma unroll
@@ -600,6 +623,13 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
         rhs[i + 1] += dot32<D>(tmc[i], S) + mu * (dm * (sn.up[i] - sp.up[i]) * dmi + dot32<D>(K, du[i]));
     rhs[D + 1] += dot32<D>(sU, S) + kcond * (dm * (aT - cT) * dmi + dot32<D>(K, dT));
     }
+}
+
+template <int D, int SOLVER, bool VISC, bool STORED>
+__device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) {
+    if (f >= a.nf) return;
+    float rhs[D + 2];
+    face_flux32_rhs<D, SOLVER, VISC, STORED>(a, f, GradGlobal32<D>{a}, a.fP[f], a.fN[f], rhs);
 #pragma unroll
     for (int k = 0; k < D + 2; k++) a.frhs[(size_t)k * a.nf + f] = rhs[k];
 }
@@ -607,6 +637,63 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
 // ---------------------------------------------------------------------------
 // ordered gather of face fluxes, sponge, RK update (deviations)
 // ---------------------------------------------------------------------------
+// Where the update reads the face fluxes (the solver's frhs array, by face)
+template <int D>
+struct FluxGlobal32 {
+    const KArgs32 &a;
+    __device__ __forceinline__ float r(int e, int f, int k) const {
+        (void)e;
+        return a.frhs[(size_t)k * a.nf + f];
+    }
+};
+// ordered gather of one cell's face fluxes, sponge and RK update from the
+// state in a.q, written to qout (a.q itself for the in-place update) with
+// the cell's boundary ghosts refreshed there; res receives the residual
+template <int D, bool RES, class FA>
+__device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &fa, float A, float B, int zero_dq,
+                                                   const int c, float *qout, float (&res)[D + 2]) {
+    const float dt = (float)(*a.dt);
+    const float vinv = a.vinv[c];
+    float dq[D + 2];
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) {
+        dq[k] = zero_dq ? 0.f : a.dq[(size_t)k * a.nc + c] * A;
+        res[k] = 0.f;
+    }
+    const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
+    int gface[kMaxFaces], ngf = 0;
+    for (int e = e0; e < e1; e++) {
+        const int ent = a.g_ent[e];
+        const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
+        if (ent >= 0 && (ent & kGhostFaceFlag) && ngf < kMaxFaces) gface[ngf++] = f;
+        const float sg = ent >= 0 ? 1.f : -1.f;
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            const float r = sg * fa.r(e, f, k);
+            if (RES) res[k] += r;
+            dq[k] += dt * r * vinv;
+        }
+    }
+    float q[D + 2];
+    load_q32<D>(a, c, q);
+    const float sgm = a.sigma[c];
+#pragma unroll
+    for (int k = 0; k < D + 2; k++) {
+        dq[k] -= dt * sgm * q[k];  // sponge: sigma (q_inf - q) = -sigma q'
+        q[k] += B * dq[k];
+        qout[(size_t)k * a.ntot + c] = q[k];
+        a.dq[(size_t)k * a.nc + c] = dq[k];
+    }
+    // refresh this cell's boundary ghosts for the next stage
+    for (int i = 0; i < ngf; i++) {
+        const int g = a.fN[gface[i]] - a.nc;
+        float qg[D + 2];
+        if (!ghost_state32<D>(a, a.ghost_role[g], q, qg)) continue;
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) qout[(size_t)k * a.ntot + a.nc + g] = qg[k];
+    }
+}
+
 // every thread of the block must call this (block-wide reduction when RES)
 template <int D, bool RES>
 __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, float A, float B, int zero_dq, const int c,
@@ -616,46 +703,8 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, float A, fl
 #pragma unroll
     for (int k = 0; k < D + 2; k++) rsq[k] = 0.0;
     if (c < a.nc) {
-        const float dt = (float)(*a.dt);
-        const float vinv = a.vinv[c];
-        float dq[D + 2], res[D + 2];
-#pragma unroll
-        for (int k = 0; k < D + 2; k++) {
-            dq[k] = zero_dq ? 0.f : a.dq[(size_t)k * a.nc + c] * A;
-            res[k] = 0.f;
-        }
-        const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
-        int gface[kMaxFaces], ngf = 0;
-        for (int e = e0; e < e1; e++) {
-            const int ent = a.g_ent[e];
-            const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
-            if (ent >= 0 && (ent & kGhostFaceFlag) && ngf < kMaxFaces) gface[ngf++] = f;
-            const float sg = ent >= 0 ? 1.f : -1.f;
-#pragma unroll
-            for (int k = 0; k < D + 2; k++) {
-                const float r = sg * a.frhs[(size_t)k * a.nf + f];
-                if (RES) res[k] += r;
-                dq[k] += dt * r * vinv;
-            }
-        }
-        float q[D + 2];
-        load_q32<D>(a, c, q);
-        const float sgm = a.sigma[c];
-#pragma unroll
-        for (int k = 0; k < D + 2; k++) {
-            dq[k] -= dt * sgm * q[k];  // sponge: sigma (q_inf - q) = -sigma q'
-            q[k] += B * dq[k];
-            a.q[(size_t)k * a.ntot + c] = q[k];
-            a.dq[(size_t)k * a.nc + c] = dq[k];
-        }
-        // refresh this cell's boundary ghosts for the next stage
-        for (int i = 0; i < ngf; i++) {
-            const int g = a.fN[gface[i]] - a.nc;
-            float qg[D + 2];
-            if (!ghost_state32<D>(a, a.ghost_role[g], q, qg)) continue;
-#pragma unroll
-            for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + a.nc + g] = qg[k];
-        }
+        float res[D + 2];
+        cell_update32_cell<D, RES>(a, FluxGlobal32<D>{a}, A, B, zero_dq, c, a.q, res);
         if (RES) {
 #pragma unroll
             for (int k = 0; k < D + 2; k++) {
