@@ -36,6 +36,8 @@ struct KArgs32 {
     float *q, *dq, *res_cell;
     float *dudx, *dTdx, *tauMC, *sigmaU;
     float *rho_grad, *p_grad, *U_grad;
+    // face fluxes: components 0-3 as one 16-byte record per face ([nf][4]),
+    // then (3D) component 4 as [nf]: one vector load per gathered face
     float *frhs;
     const double *dt;
     double *partial;
@@ -49,6 +51,14 @@ struct KArgs32 {
 };
 
 #define LERP32(w, a, b) ((w) * (a) + (1.0f - (w)) * (b))
+
+// Faces per cell whose gathers are issued together before any arithmetic
+// (the cell's faces beyond it are gathered one at a time): every cell of a
+// 2D quad/triangle mesh and of a 3D hex mesh fits.
+template <int D>
+struct GatherUnroll {
+    static constexpr int value = D == 2 ? 4 : 6;
+};
 
 // FP32 reciprocal, quotient and square root on the hardware instructions
 // (v_rcp_f32, v_sqrt_f32: 1 ulp).  Spelled as target builtins so that every
@@ -261,18 +271,8 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
     float qc0[D + 2];
     load_q32<D>(a, c, qc0);
     const Prim32<D> own = prim32<D>(a, qc0);
-    const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
-    for (int e = e0; e < e1; e++) {
-        const int ent = a.g_ent[e];
-        const bool isP = ent >= 0;
-        const int f = isP ? (ent & kFaceMask) : ~ent;
-        // the face's other cell; the face values use the face's own P/N order
-        const int other = isP ? a.fN[f] : a.fP[f];
-        const float w = a.w[f];
-        float S[D], qo[D + 2];
-#pragma unroll
-        for (int k = 0; k < D; k++) S[k] = a.S[(size_t)k * a.nf + f];
-        load_q32<D>(a, other, qo);
+    // one face's contribution (the face values use the face's own P/N order)
+    auto add = [&](const bool isP, const float w, const float (&S)[D], const float (&qo)[D + 2]) {
         const Prim32<D> so = prim32<D>(a, qo);
         const Prim32<D> &sp = isP ? own : so;
         const Prim32<D> &sn = isP ? so : own;
@@ -300,6 +300,36 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
                 for (int j = 0; j < D; j++) ug[j][i] += fU[j] * sv[i];
             }
         }
+    };
+    const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
+    // the first kGatherUnroll faces: every load issued before the sums (a
+    // short list repeats its last face, whose terms are then not added)
+    constexpr int U = GatherUnroll<D>::value;
+    bool isP[U];
+    float w[U], S[U][D], qo[U][D + 2];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+        const int ent = a.g_ent[e0 + (j < ne ? j : ne - 1)];
+        isP[j] = ent >= 0;
+        const int f = isP[j] ? (ent & kFaceMask) : ~ent;
+        const int other = isP[j] ? a.fN[f] : a.fP[f];
+        w[j] = a.w[f];
+#pragma unroll
+        for (int k = 0; k < D; k++) S[j][k] = a.S[(size_t)k * a.nf + f];
+        load_q32<D>(a, other, qo[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++)
+        if (j < ne) add(isP[j], w[j], S[j], qo[j]);
+    for (int e = e0 + U; e < e0 + ne; e++) {
+        const int ent = a.g_ent[e];
+        const bool p = ent >= 0;
+        const int f = p ? (ent & kFaceMask) : ~ent;
+        float Sf[D], q[D + 2];
+#pragma unroll
+        for (int k = 0; k < D; k++) Sf[k] = a.S[(size_t)k * a.nf + f];
+        load_q32<D>(a, p ? a.fN[f] : a.fP[f], q);
+        add(p, a.w[f], Sf, q);
     }
     return own;
 }
@@ -630,8 +660,8 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
     if (f >= a.nf) return;
     float rhs[D + 2];
     face_flux32_rhs<D, SOLVER, VISC, STORED>(a, f, GradGlobal32<D>{a}, a.fP[f], a.fN[f], rhs);
-#pragma unroll
-    for (int k = 0; k < D + 2; k++) a.frhs[(size_t)k * a.nf + f] = rhs[k];
+    reinterpret_cast<float4 *>(a.frhs)[f] = make_float4(rhs[0], rhs[1], rhs[2], rhs[3]);
+    if (D == 3) a.frhs[(size_t)4 * a.nf + f] = rhs[D + 1];
 }
 
 // ---------------------------------------------------------------------------
@@ -641,9 +671,14 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
 template <int D>
 struct FluxGlobal32 {
     const KArgs32 &a;
-    __device__ __forceinline__ float r(int e, int f, int k) const {
+    __device__ __forceinline__ void get(int e, int f, float (&r)[D + 2]) const {
         (void)e;
-        return a.frhs[(size_t)k * a.nf + f];
+        const float4 v = reinterpret_cast<const float4 *>(a.frhs)[f];
+        r[0] = v.x;
+        r[1] = v.y;
+        r[2] = v.z;
+        r[3] = v.w;
+        if (D == 3) r[D + 1] = a.frhs[(size_t)4 * a.nf + f];
     }
 };
 // ordered gather of one cell's face fluxes, sponge and RK update from the
@@ -660,19 +695,37 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
         dq[k] = zero_dq ? 0.f : a.dq[(size_t)k * a.nc + c] * A;
         res[k] = 0.f;
     }
-    const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
+    const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
     int gface[kMaxFaces], ngf = 0;
-    for (int e = e0; e < e1; e++) {
-        const int ent = a.g_ent[e];
+    auto add = [&](const int ent, const float (&fr)[D + 2]) {
         const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
         if (ent >= 0 && (ent & kGhostFaceFlag) && ngf < kMaxFaces) gface[ngf++] = f;
         const float sg = ent >= 0 ? 1.f : -1.f;
 #pragma unroll
         for (int k = 0; k < D + 2; k++) {
-            const float r = sg * fa.r(e, f, k);
+            const float r = sg * fr[k];
             if (RES) res[k] += r;
             dq[k] += dt * r * vinv;
         }
+    };
+    // the first faces' fluxes fetched together, then added in gather order
+    constexpr int U = GatherUnroll<D>::value;
+    int ent[U];
+    float fr[U][D + 2];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+        const int e = e0 + (j < ne ? j : ne - 1);
+        ent[j] = a.g_ent[e];
+        fa.get(e, ent[j] >= 0 ? (ent[j] & kFaceMask) : ~ent[j], fr[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++)
+        if (j < ne) add(ent[j], fr[j]);
+    for (int e = e0 + U; e < e0 + ne; e++) {
+        const int en = a.g_ent[e];
+        float f1[D + 2];
+        fa.get(e, en >= 0 ? (en & kFaceMask) : ~en, f1);
+        add(en, f1);
     }
     float q[D + 2];
     load_q32<D>(a, c, q);
