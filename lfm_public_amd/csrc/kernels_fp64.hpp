@@ -282,17 +282,9 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
     double qself[D + 2];
     load_q<D>(a, c, qself);
     const double Rself = rpsi<D>(a, qself), riself = 1.0 / qself[0];
-    const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
-    for (int e = e0; e < e1; e++) {
-        const int ent = a.g_ent[e];
-        const bool isP = ent >= 0;
-        const int f = isP ? (ent & kFaceMask) : ~ent;
-        const int other = isP ? a.fN[f] : a.fP[f];
-        const double w = a.w[f];
-        double S[D], qo[D + 2], qP[D + 2], qN[D + 2];
-#pragma unroll
-        for (int k = 0; k < D; k++) S[k] = a.S[(size_t)k * a.nf + f];
-        load_q<D>(a, other, qo);
+    // one face's contribution, in the face's own P/N order
+    auto add = [&](const bool isP, const double w, const double (&S)[D], const double (&qo)[D + 2]) {
+        double qP[D + 2], qN[D + 2];
         const double Ro = rpsi<D>(a, qo), rio = 1.0 / qo[0];
 #pragma unroll
         for (int k = 0; k < D + 2; k++) {
@@ -338,6 +330,35 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
             for (int j = 0; j < D; j++) dudx[i][j] += fU[i] * sv[j];
             if (VISC) dTdx[i] += fT * sv[i];
         }
+    };
+    const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
+    // the first faces' gathers issued together before the sums (a short list
+    // repeats its last face, whose terms are then not added)
+    constexpr int U = D == 2 ? 4 : 6;
+    bool isPj[U];
+    double wj[U], Sj[U][D], qj[U][D + 2];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+        const int ent = a.g_ent[e0 + (j < ne ? j : ne - 1)];
+        isPj[j] = ent >= 0;
+        const int f = isPj[j] ? (ent & kFaceMask) : ~ent;
+        wj[j] = a.w[f];
+#pragma unroll
+        for (int k = 0; k < D; k++) Sj[j][k] = a.S[(size_t)k * a.nf + f];
+        load_q<D>(a, isPj[j] ? a.fN[f] : a.fP[f], qj[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++)
+        if (j < ne) add(isPj[j], wj[j], Sj[j], qj[j]);
+    for (int e = e0 + U; e < e0 + ne; e++) {
+        const int ent = a.g_ent[e];
+        const bool p = ent >= 0;
+        const int f = p ? (ent & kFaceMask) : ~ent;
+        double Sf[D], q[D + 2];
+#pragma unroll
+        for (int k = 0; k < D; k++) Sf[k] = a.S[(size_t)k * a.nf + f];
+        load_q<D>(a, p ? a.fN[f] : a.fP[f], q);
+        add(p, a.w[f], Sf, q);
     }
     // refresh of tau / sigmaU / tauMC from the completed dudx (cfd_v0.cpp:1813-1834);
     // stored only where a halo payload needs them (otherwise each face
@@ -706,28 +727,46 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
             dq[k] = zero_dq ? 0.0 : a.dq[(size_t)k * a.nc + c] * A;
             res[k] = 0.0;
         }
-        const int e0 = a.g_off[c], e1 = a.g_off[c + 1];
+        const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
         int gface[kMaxFaces], ngf = 0;
-        for (int e = e0; e < e1; e++) {
-            const int ent = a.g_ent[e];
+        // owner side adds, neighbour side subtracts (cfd_v0.cpp:2800-2806)
+        auto add = [&](const int ent, const double (&fr)[D + 2]) {
             if (ent >= 0) {
-                const int f = ent & kFaceMask;
-                if ((ent & kGhostFaceFlag) && ngf < kMaxFaces) gface[ngf++] = f;
+                if ((ent & kGhostFaceFlag) && ngf < kMaxFaces) gface[ngf++] = ent & kFaceMask;
 #pragma unroll
                 for (int k = 0; k < D + 2; k++) {
-                    const double r = a.frhs[(size_t)k * a.nf + f];
-                    if (RES) res[k] += r;
-                    dq[k] += dt * r * vinv;
+                    if (RES) res[k] += fr[k];
+                    dq[k] += dt * fr[k] * vinv;
                 }
             } else {
-                const int f = ~ent;
 #pragma unroll
                 for (int k = 0; k < D + 2; k++) {
-                    const double r = a.frhs[(size_t)k * a.nf + f];
-                    if (RES) res[k] -= r;
-                    dq[k] -= dt * r * vinv;
+                    if (RES) res[k] -= fr[k];
+                    dq[k] -= dt * fr[k] * vinv;
                 }
             }
+        };
+        // the first faces' fluxes fetched together, then added in gather order
+        constexpr int U = D == 2 ? 4 : 6;
+        int ent[U];
+        double fr[U][D + 2];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            ent[j] = a.g_ent[e0 + (j < ne ? j : ne - 1)];
+            const int f = ent[j] >= 0 ? (ent[j] & kFaceMask) : ~ent[j];
+#pragma unroll
+            for (int k = 0; k < D + 2; k++) fr[j][k] = a.frhs[(size_t)k * a.nf + f];
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++)
+            if (j < ne) add(ent[j], fr[j]);
+        for (int e = e0 + U; e < e0 + ne; e++) {
+            const int en = a.g_ent[e];
+            const int f = en >= 0 ? (en & kFaceMask) : ~en;
+            double f1[D + 2];
+#pragma unroll
+            for (int k = 0; k < D + 2; k++) f1[k] = a.frhs[(size_t)k * a.nf + f];
+            add(en, f1);
         }
         double q[D + 2];
         load_q<D>(a, c, q);
