@@ -129,13 +129,21 @@ int Solver::set_field(int which, const double *in) {
         KArgs32 &a = *k32;
         double qinf[5] = {a.rho_inf_d, a.m_inf_d[0], a.m_inf_d[1], a.m_inf_d[2], 0.0};
         qinf[L.D + 1] = a.rhoE_inf_d;
-        std::vector<float> tmp(nc);
+        // cell records: components 0-3 interleaved, component 4 after them
+        const size_t nt = (size_t)L.nc + L.ng;
+        std::vector<float> rec((size_t)4 * nc), e4(L.D == 3 ? nc : 0);
         for (int k = 0; k < L.D + 2; k++) {
             const double *src = in + k * nc;
-            for (size_t ci = 0; ci < nc; ci++) tmp[ci] = (float)(src[L.orig[ci]] - qinf[k]);
-            HIP_TRY(hipMemcpy(a.q + k * ((size_t)L.nc + L.ng), tmp.data(), nc * sizeof(float),
-                              hipMemcpyHostToDevice));
+            for (size_t ci = 0; ci < nc; ci++) {
+                const float v = (float)(src[L.orig[ci]] - qinf[k]);
+                if (k < 4)
+                    rec[4 * ci + k] = v;
+                else
+                    e4[ci] = v;
+            }
         }
+        HIP_TRY(hipMemcpy(a.q, rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice));
+        if (L.D == 3) HIP_TRY(hipMemcpy(a.q + 4 * nt, e4.data(), nc * sizeof(float), hipMemcpyHostToDevice));
         return LFM_OK;
     }
     FieldRef r = field_ref64(this, which);

@@ -94,10 +94,27 @@ __device__ __forceinline__ Prim32<D> prim32(const KArgs32 &a, const float *q) {
     return s;
 }
 
+// FP32 state layout: components 0-3 of a cell (or ghost) i as one 16-byte
+// record, q[4 i + k], then component 4 (the 3D energy) as q[4 ntot + i]: a
+// gathered neighbour state is one vector load (two in 3D).
+__host__ __device__ __forceinline__ size_t q32_index(int k, size_t i, size_t ntot) {
+    return k < 4 ? 4 * i + (size_t)k : 4 * ntot + i;
+}
+
 template <int D>
 __device__ __forceinline__ void load_q32(const KArgs32 &a, int i, float *q) {
-#pragma unroll
-    for (int k = 0; k < D + 2; k++) q[k] = a.q[(size_t)k * a.ntot + i];
+    const float4 v = reinterpret_cast<const float4 *>(a.q)[i];
+    q[0] = v.x;
+    q[1] = v.y;
+    q[2] = v.z;
+    q[3] = v.w;
+    if (D == 3) q[4] = a.q[(size_t)4 * a.ntot + i];
+}
+
+template <int D>
+__device__ __forceinline__ void store_q32(float *qv, size_t ntot, int i, const float *q) {
+    reinterpret_cast<float4 *>(qv)[i] = make_float4(q[0], q[1], q[2], q[3]);
+    if (D == 3) qv[4 * ntot + i] = q[4];
 }
 
 template <int D>
@@ -195,8 +212,7 @@ __device__ __forceinline__ void bc_fill32_body(const KArgs32 &a, const int g) {
     load_q32<D>(a, a.ghost_cell[g], qi);
     if (!ghost_state32<D>(a, a.ghost_role[g], qi, qg)) return;
     const int gi = a.nc + g;
-#pragma unroll
-    for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + gi] = qg[k];
+    store_q32<D>(a.q, a.ntot, gi, qg);
 }
 
 // ---------------------------------------------------------------------------
@@ -734,16 +750,15 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
     for (int k = 0; k < D + 2; k++) {
         dq[k] -= dt * sgm * q[k];  // sponge: sigma (q_inf - q) = -sigma q'
         q[k] += B * dq[k];
-        qout[(size_t)k * a.ntot + c] = q[k];
         a.dq[(size_t)k * a.nc + c] = dq[k];
     }
+    store_q32<D>(qout, a.ntot, c, q);
     // refresh this cell's boundary ghosts for the next stage
     for (int i = 0; i < ngf; i++) {
         const int g = a.fN[gface[i]] - a.nc;
         float qg[D + 2];
         if (!ghost_state32<D>(a, a.ghost_role[g], q, qg)) continue;
-#pragma unroll
-        for (int k = 0; k < D + 2; k++) qout[(size_t)k * a.ntot + a.nc + g] = qg[k];
+        store_q32<D>(qout, a.ntot, a.nc + g, qg);
     }
 }
 
@@ -793,10 +808,10 @@ __device__ __forceinline__ void cfl_dt_cells32_body(const KArgs32 &a, double par
     __shared__ double sh[256];
     double v = MODE == 0 ? 100.0 : 0.0;
     if (c < a.nc) {
-        const double rho = a.rho_inf_d + (double)a.q[c];
+        const double rho = a.rho_inf_d + (double)a.q[q32_index(0, c, a.ntot)];
         double m[D];
 #pragma unroll
-        for (int k = 0; k < D; k++) m[k] = a.m_inf_d[k] + (double)a.q[(size_t)(k + 1) * a.ntot + c];
+        for (int k = 0; k < D; k++) m[k] = a.m_inf_d[k] + (double)a.q[q32_index(k + 1, c, a.ntot)];
         double sum = 0.0;
         for (int e = a.s_off[c]; e < a.s_off[c + 1]; e++) {
             const int ent = a.s_ent[e];
@@ -856,10 +871,10 @@ __global__ void __launch_bounds__(256) integrals32(KArgs32 a) {
     for (int k = 0; k < D + 2; k++) v[k] = 0.0;
     if (c < a.nc) {
         const double vol = a.vol_d[c];
-        v[0] = vol * (a.rho_inf_d + (double)a.q[c]);
+        v[0] = vol * (a.rho_inf_d + (double)a.q[q32_index(0, c, a.ntot)]);
 #pragma unroll
-        for (int k = 0; k < D; k++) v[k + 1] = vol * (a.m_inf_d[k] + (double)a.q[(size_t)(k + 1) * a.ntot + c]);
-        v[D + 1] = vol * (a.rhoE_inf_d + (double)a.q[(size_t)(D + 1) * a.ntot + c]);
+        for (int k = 0; k < D; k++) v[k + 1] = vol * (a.m_inf_d[k] + (double)a.q[q32_index(k + 1, c, a.ntot)]);
+        v[D + 1] = vol * (a.rhoE_inf_d + (double)a.q[q32_index(D + 1, c, a.ntot)]);
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -885,7 +900,7 @@ __global__ void __launch_bounds__(256) count_nonfinite32(const float *q, int nc,
     if (c >= nc) return;
     int bad = 0;
 #pragma unroll
-    for (int k = 0; k < D + 2; k++) bad += !isfinite(q[(size_t)k * ntot + c]);
+    for (int k = 0; k < D + 2; k++) bad += !isfinite(q[q32_index(k, c, ntot)]);
     if (bad) atomicAdd(cnt, (unsigned long long)bad);
 }
 

@@ -5,9 +5,10 @@
 //   SOLVARS: q[0..D+1]
 //   VISCOUS: dudx[i][j] (row-major), dTdx[i], tauMC[i][j], sigmaU[i]
 //   PACKED : SOLVARS then VISCOUS
-// Device fields are SoA [component][ntot], so component m of a record reads
-// field[m * ntot + cell]: one thread per (record, component) keeps the
-// buffer writes (pack) and reads (unpack) contiguous.
+// Device fields are SoA [component][ntot] (component m of a record reads
+// field[m * ntot + cell]), except the FP32 state, stored as cell records:
+// one thread per (record, component) keeps the buffer writes (pack) and
+// reads (unpack) contiguous.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,7 +21,15 @@ template <typename T>
 struct HaloFields {
     T *q, *dudx, *dTdx, *tauMC, *sigmaU;
     int ntot, D, kind;  // kind: bit 0 SOLVARS, bit 1 VISCOUS
+    int q_records;      // q stored as 16-byte cell records (the FP32 layout)
 };
+
+// index of component m of cell i in field f
+template <typename T>
+__device__ __forceinline__ size_t halo_index(const HaloFields<T> &h, const T *f, int m, int i) {
+    if (f == h.q && h.q_records) return m < 4 ? 4 * (size_t)i + m : 4 * (size_t)h.ntot + i;
+    return (size_t)m * h.ntot + i;
+}
 
 template <typename T>
 __device__ __forceinline__ T *halo_component(const HaloFields<T> &h, int k, int &m) {
@@ -58,7 +67,7 @@ __global__ void __launch_bounds__(256) halo_pack_kernel(HaloFields<T> h, const i
     const int i = (int)(t / nrec), k = (int)(t % nrec);
     int m;
     const T *src = halo_component(h, k, m);
-    buf[t] = src[(size_t)m * h.ntot + cells[i]];
+    buf[t] = src[halo_index(h, src, m, cells[i])];
 }
 
 // component k of ghost slot ghost[j] = buf[src[j] * nrec + k]
@@ -70,7 +79,7 @@ __global__ void __launch_bounds__(256) halo_unpack_kernel(HaloFields<T> h, const
     const int j = (int)(t / nrec), k = (int)(t % nrec);
     int m;
     T *dst = halo_component(h, k, m);
-    dst[(size_t)m * h.ntot + ghost[j]] = buf[(size_t)src[j] * nrec + k];
+    dst[halo_index(h, dst, m, ghost[j])] = buf[(size_t)src[j] * nrec + k];
 }
 
 // dt = min over partitions of their dt minima (MPI_Allreduce MIN,

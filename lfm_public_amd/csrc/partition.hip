@@ -51,6 +51,7 @@ static HaloFields<T> fields_of(T *q, T *dudx, T *dTdx, T *tau, T *sU, int ntot, 
     h.ntot = ntot;
     h.D = D;
     h.kind = kind;
+    h.q_records = sizeof(T) == 4;  // the FP32 state is stored as cell records
     return h;
 }
 

@@ -119,8 +119,8 @@ int create_fp32p(Solver *s) {
         for (int k = 0; k < D; k++) qi[k + 1] = qinf[k + 1];
         qi[D + 1] = rhoE_inf;
         for (int k = 0; k < D + 2; k++) {
-            for (size_t c = 0; c < nc; c++) q0[k * ntot + c] = (float)(L.q0[k * nc + c] - qi[k]);
-            for (size_t gi = 0; gi < (size_t)L.ng; gi++) q0[k * ntot + nc + gi] = (float)(-qi[k]);
+            for (size_t c = 0; c < nc; c++) q0[q32_index(k, c, ntot)] = (float)(L.q0[k * nc + c] - qi[k]);
+            for (size_t gi = 0; gi < (size_t)L.ng; gi++) q0[q32_index(k, nc + gi, ntot)] = (float)(-qi[k]);
         }
         rc |= up(s, &a.q, q0.data(), q0.size());
     }
@@ -287,6 +287,17 @@ int get_field32(Solver *s, int which, double *out) {
     }
     if (!ptr) return fail(LFM_EINVAL, "field not allocated for this solver");
     double qinf[5] = {a.rho_inf_d, a.m_inf_d[0], a.m_inf_d[1], a.m_inf_d[2], 0.0};
+    if (which == LFM_FIELD_Q) {
+        // cell records: components 0-3 interleaved, component 4 after them
+        qinf[D + 1] = a.rhoE_inf_d;
+        std::vector<float> rec((size_t)4 * nc), e4(D == 3 ? nc : 0);
+        HIP_TRY(hipMemcpy(rec.data(), a.q, rec.size() * sizeof(float), hipMemcpyDeviceToHost));
+        if (D == 3) HIP_TRY(hipMemcpy(e4.data(), a.q + 4 * nt, nc * sizeof(float), hipMemcpyDeviceToHost));
+        for (int k = 0; k < D + 2; k++)
+            for (size_t ci = 0; ci < nc; ci++)
+                out[k * nc + L.orig[ci]] = qinf[k] + (double)(k < 4 ? rec[4 * ci + k] : e4[ci]);
+        return LFM_OK;
+    }
     qinf[D + 1] = a.rhoE_inf_d;
     std::vector<float> tmp(nc);
     for (int k = 0; k < ncomp; k++) {
