@@ -53,10 +53,13 @@ static void enqueue_step(Batch *b) {
         // without viscosity solvers 0/1 consume no gradient or tensor
         if (V || b->solver == 2) {
             if (b->solver == 2 && b->minmod)
-                hipLaunchKernelGGL((grad_vis32_b<D, 2, V>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
+                hipLaunchKernelGGL((grad_vis32_b<D, 2, V, true>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
+                                   b->map_cells);
+            else if (b->solver == 2)
+                hipLaunchKernelGGL((grad_vis32_b<D, 0, V, true>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
                                    b->map_cells);
             else
-                hipLaunchKernelGGL((grad_vis32_b<D, 0, V>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
+                hipLaunchKernelGGL((grad_vis32_b<D, 0, V, false>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
                                    b->map_cells);
         }
         switch (b->solver) {

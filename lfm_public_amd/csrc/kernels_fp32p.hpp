@@ -34,8 +34,13 @@ struct KArgs32 {
     const float *vinv, *sigma;
     const float *smag;  // [nc] Smagorinsky constants, nullptr when laminar
     float *q, *dq, *res_cell;
+    // per-cell gradient records (GradRec32): dudx points at the records,
+    // dTdx / rho_grad / p_grad into them (component stride 1, cell stride
+    // gstride); tauMC / sigmaU (partitions only) and a U_grad that is not
+    // dudx are [component][ntot]
     float *dudx, *dTdx, *tauMC, *sigmaU;
     float *rho_grad, *p_grad, *U_grad;
+    int gstride;
     // face fluxes: components 0-3 as one 16-byte record per face ([nf][4]),
     // then (3D) component 4 as [nf]: one vector load per gathered face
     float *frhs;
@@ -271,6 +276,23 @@ __device__ __forceinline__ void vis_tensors32(const KArgs32 &a, const float (&du
     for (int i = 0; i < D; i++) sU[i] = dot32<D>(u, tau[i]);
 }
 
+// One cell's gradients as one record, padded to whole 16-byte words: dudx
+// (D*D), dTdx (D), then with RG (the M2AUSM scheme) the rho and p gradients
+// (D each); ghosts' records stay zero.  A gathered neighbour's gradients are
+// a few vector loads.
+template <int D, bool RG>
+struct GradRec32 {
+    static constexpr int kDT = D * D;
+    static constexpr int kRG = kDT + D;
+    static constexpr int kPG = kRG + D;
+    static constexpr int kN = RG ? kPG + D : kRG;
+    static constexpr int kStride = (kN + 3) / 4 * 4;
+};
+inline int grad_record_stride(int D, bool rg) {
+    const int n = D * D + D + (rg ? 2 * D : 0);
+    return (n + 3) / 4 * 4;
+}
+
 // Green-Gauss sums of one cell over its faces in gather order; returns the
 // cell's own primitive state
 template <int D, int GRAD, bool VISC>
@@ -350,43 +372,77 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
     return own;
 }
 
-template <int D, int GRAD, bool VISC>
+// every thread of the block must call this (the block's records are written
+// through LDS); c = the block's first cell + threadIdx.x
+template <int D, int GRAD, bool VISC, bool RG>
 __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
-    if (c >= a.nc) return;
-    float dudx[D][D], dTdx[D], rg[D], pg[D], ug[D][D];
-    const Prim32<D> own = grad32_sums<D, GRAD, VISC>(a, c, dudx, dTdx, rg, pg, ug);
-    // tauMC / sigmaU stored only where a halo payload needs them (otherwise
-    // each face recomputes them from dudx); U_grad equals dudx bit for bit
-    // (same face velocity, same sums) and shares its storage unless a halo
-    // payload must leave it zero on the ghosts
-    const size_t nt = a.ntot;
-    if (VISC && a.tauMC) {
-        float tmc[D][D], sU[D];
-        vis_tensors32<D>(a, dudx, own.u, a.smag ? a.smag + c : nullptr, tmc, sU);
+    using R = GradRec32<D, RG>;
+    const bool valid = c < a.nc;
+    float r[R::kStride];
+#pragma unroll
+    for (int k = 0; k < R::kStride; k++) r[k] = 0.f;
+    if (valid) {
+        float dudx[D][D], dTdx[D], rg[D], pg[D], ug[D][D];
+        const Prim32<D> own = grad32_sums<D, GRAD, VISC>(a, c, dudx, dTdx, rg, pg, ug);
+        // tauMC / sigmaU stored only where a halo payload needs them (otherwise
+        // each face recomputes them from dudx); U_grad equals dudx bit for bit
+        // (same face velocity, same sums) and shares its storage unless a halo
+        // payload must leave it zero on the ghosts
+        const size_t nt = a.ntot;
+        if (VISC && a.tauMC) {
+            float tmc[D][D], sU[D];
+            vis_tensors32<D>(a, dudx, own.u, a.smag ? a.smag + c : nullptr, tmc, sU);
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+                a.sigmaU[i * nt + c] = sU[i];
+#pragma unroll
+                for (int j = 0; j < D; j++) a.tauMC[(i * D + j) * nt + c] = tmc[i][j];
+            }
+        }
 #pragma unroll
         for (int i = 0; i < D; i++) {
-            a.sigmaU[i * nt + c] = sU[i];
 #pragma unroll
-            for (int j = 0; j < D; j++) a.tauMC[(i * D + j) * nt + c] = tmc[i][j];
+            for (int j = 0; j < D; j++) r[i * D + j] = dudx[i][j];
+            r[R::kDT + i] = VISC ? dTdx[i] : 0.f;
+            if (RG && GRAD == 2) {
+                r[R::kRG + i] = rg[i];
+                r[R::kPG + i] = pg[i];
+            }
         }
-    }
-#pragma unroll
-    for (int i = 0; i < D; i++) {
-#pragma unroll
-        for (int j = 0; j < D; j++) a.dudx[(i * D + j) * nt + c] = dudx[i][j];
-        if (VISC) a.dTdx[i * nt + c] = dTdx[i];
-    }
-    if (GRAD == 2) {
-#pragma unroll
-        for (int i = 0; i < D; i++) {
-            a.rho_grad[i * nt + c] = rg[i];
-            a.p_grad[i * nt + c] = pg[i];
-        }
-        if (a.U_grad != a.dudx)
+        if (GRAD == 2 && a.U_grad != a.dudx)
 #pragma unroll
             for (int i = 0; i < D; i++)
 #pragma unroll
                 for (int j = 0; j < D; j++) a.U_grad[(i * D + j) * nt + c] = ug[i][j];
+        if (!VISC) {
+            // dTdx not formed: the formed components only (the record keeps
+            // whatever a full calc_VIS left in the dTdx slots)
+            float *rec = a.dudx + (size_t)c * R::kStride;
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+#pragma unroll
+                for (int j = 0; j < D; j++) rec[i * D + j] = r[i * D + j];
+                if (RG && GRAD == 2) {
+                    rec[R::kRG + i] = r[R::kRG + i];
+                    rec[R::kPG + i] = r[R::kPG + i];
+                }
+            }
+        }
+    }
+    if (VISC) {
+        // the block's records are contiguous from its first cell: staged in
+        // LDS, then written as consecutive 16-byte words (without minmod the
+        // rho / p slots are written zero, as they stay)
+        constexpr int W = R::kStride / 4;
+        __shared__ float4 st[256 * W];
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < W; k++) st[t * W + k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+        __syncthreads();
+        const int c0 = c - t;
+        const int nrec = a.nc - c0 < (int)blockDim.x ? a.nc - c0 : (int)blockDim.x;
+        float4 *dst = reinterpret_cast<float4 *>(a.dudx + (size_t)c0 * R::kStride);
+        for (int j = t; j < nrec * W; j += blockDim.x) dst[j] = st[j];
     }
 }
 
@@ -394,14 +450,26 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
 // Where the face kernel reads the cells' gradients (the solver's arrays,
 // indexed by cell)
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool RG>
 struct GradGlobal32 {
     const KArgs32 &a;
-    __device__ __forceinline__ float du(int c, int i, int j) const { return a.dudx[(size_t)(i * D + j) * a.ntot + c]; }
-    __device__ __forceinline__ float dT(int c, int i) const { return a.dTdx[(size_t)i * a.ntot + c]; }
-    __device__ __forceinline__ float rg(int c, int i) const { return a.rho_grad[(size_t)i * a.ntot + c]; }
-    __device__ __forceinline__ float pg(int c, int i) const { return a.p_grad[(size_t)i * a.ntot + c]; }
-    __device__ __forceinline__ float ug(int c, int i, int j) const { return a.U_grad[(size_t)(i * D + j) * a.ntot + c]; }
+    using R = GradRec32<D, RG>;
+    // the gradient record of cell c
+    __device__ __forceinline__ void rec(int c, float (&r)[R::kStride]) const {
+        const float4 *p = reinterpret_cast<const float4 *>(a.dudx + (size_t)c * R::kStride);
+#pragma unroll
+        for (int k = 0; k < R::kStride / 4; k++) {
+            const float4 v = p[k];
+            r[4 * k] = v.x;
+            r[4 * k + 1] = v.y;
+            r[4 * k + 2] = v.z;
+            r[4 * k + 3] = v.w;
+        }
+    }
+    // U_grad: dudx itself, or its own [component][ntot] array
+    __device__ __forceinline__ float ug(const float (&r)[R::kStride], int c, int i, int j) const {
+        return a.U_grad == a.dudx ? r[i * D + j] : a.U_grad[(size_t)(i * D + j) * a.ntot + c];
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -411,6 +479,12 @@ template <int D, int SOLVER, bool VISC, bool STORED, class GA>
 __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, const GA &ga, const int gP,
                                                 const int gN, float (&rhs)[D + 2]) {
     const int P = a.fP[f], N = a.fN[f];
+    using R = GradRec32<D, SOLVER == 2>;
+    float gr[2][R::kStride];  // gradient records of P and N
+    if (VISC || SOLVER == 2) {
+        ga.rec(gP, gr[0]);
+        ga.rec(gN, gr[1]);
+    }
     // is_ghost: boundary ghosts only; halo ghosts carry the neighbour's tensors
     const bool ghost = (unsigned)(N - a.nc) < (unsigned)a.ng_bnd;
     const float w = a.w[f];
@@ -496,8 +570,8 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
             for (int i = 0; i < D; i++)
 #pragma unroll
                 for (int j = 0; j < D; j++) {
-                    cdu[i][j] = ga.du(gP, i, j);
-                    ndu[i][j] = ga.du(gN, i, j);
+                    cdu[i][j] = gr[0][i * D + j];
+                    ndu[i][j] = gr[1][i * D + j];
                 }
 #pragma unroll
             for (int p = 0; p < D; p++) {
@@ -511,8 +585,8 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
                 float gc[D], gn[D];
 #pragma unroll
                 for (int k = 0; k < D; k++) {
-                    gc[k] = ga.ug(gP, p, k);
-                    gn[k] = ga.ug(gN, p, k);
+                    gc[k] = ga.ug(gr[0], gP, p, k);
+                    gn[k] = ga.ug(gr[1], gN, p, k);
                 }
                 unP += minmod32<D>(sp.up[p], sn.up[p], gc, d, w, 1.0f) * S[p];
                 unN += minmod32<D>(sp.up[p], sn.up[p], gn, d, w, -1.0f) * S[p];
@@ -523,10 +597,10 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
             float gc[D], gn[D], hc[D], hn[D];
 #pragma unroll
             for (int k = 0; k < D; k++) {
-                gc[k] = ga.rg(gP, k);
-                gn[k] = ga.rg(gN, k);
-                hc[k] = ga.pg(gP, k);
-                hn[k] = ga.pg(gN, k);
+                gc[k] = gr[0][R::kRG + k];
+                gn[k] = gr[1][R::kRG + k];
+                hc[k] = gr[0][R::kPG + k];
+                hn[k] = gr[1][R::kPG + k];
             }
             const float rhoPos = a.rho_inf + minmod32<D>(sp.rp, sn.rp, gc, d, w, 1.0f);
             const float rhoNeg = a.rho_inf + minmod32<D>(sp.rp, sn.rp, gn, d, w, -1.0f);
@@ -595,11 +669,11 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
         for (int i = 0; i < D; i++) {
 #pragma unroll
             for (int j = 0; j < D; j++) {
-                duP[i][j] = ga.du(gP, i, j);
-                duN[i][j] = ga.du(gN, i, j);
+                duP[i][j] = gr[0][i * D + j];
+                duN[i][j] = gr[1][i * D + j];
                 du[i][j] = LERP32(w, duP[i][j], duN[i][j]);
             }
-            dT[i] = LERP32(w, ga.dT(gP, i), ga.dT(gN, i));
+            dT[i] = LERP32(w, gr[0][R::kDT + i], gr[1][R::kDT + i]);
         }
     }
 #pragma unroll
@@ -675,7 +749,7 @@ template <int D, int SOLVER, bool VISC, bool STORED>
 __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) {
     if (f >= a.nf) return;
     float rhs[D + 2];
-    face_flux32_rhs<D, SOLVER, VISC, STORED>(a, f, GradGlobal32<D>{a}, a.fP[f], a.fN[f], rhs);
+    face_flux32_rhs<D, SOLVER, VISC, STORED>(a, f, GradGlobal32<D, SOLVER == 2>{a}, a.fP[f], a.fN[f], rhs);
     reinterpret_cast<float4 *>(a.frhs)[f] = make_float4(rhs[0], rhs[1], rhs[2], rhs[3]);
     if (D == 3) a.frhs[(size_t)4 * a.nf + f] = rhs[D + 1];
 }
@@ -911,9 +985,9 @@ template <int D>
 __global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
     bc_fill32_body<D>(a, LFM_TID(blockIdx.x));
 }
-template <int D, int GRAD, bool VISC>
+template <int D, int GRAD, bool VISC, bool RG>
 __global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
-    grad_vis32_body<D, GRAD, VISC>(a, LFM_TID(blockIdx.x));
+    grad_vis32_body<D, GRAD, VISC, RG>(a, LFM_TID(blockIdx.x));
 }
 template <int D, int SOLVER, bool VISC, bool STORED>
 __global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
@@ -939,7 +1013,7 @@ __global__ void __launch_bounds__(256) vis_tensor_field32(KArgs32 a, float *tmc_
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll
-        for (int j = 0; j < D; j++) du[i][j] = a.dudx[(size_t)(i * D + j) * a.ntot + c];
+        for (int j = 0; j < D; j++) du[i][j] = a.dudx[(size_t)c * a.gstride + i * D + j];
     load_q32<D>(a, c, q);
     const Prim32<D> s = prim32<D>(a, q);
     vis_tensors32<D>(a, du, s.u, a.smag ? a.smag + c : nullptr, tmc, sU);
@@ -968,11 +1042,11 @@ __device__ __forceinline__ void batch_slot(const BatchMap &m, int &r, int &lblk)
     lblk = b - m.first[r];
 }
 
-template <int D, int GRAD, bool VISC>
+template <int D, int GRAD, bool VISC, bool RG>
 __global__ void __launch_bounds__(256) grad_vis32_b(const KArgs32 *args, BatchMap m) {
     int r, lb;
     batch_slot(m, r, lb);
-    grad_vis32_body<D, GRAD, VISC>(args[r], lb * 256 + threadIdx.x);
+    grad_vis32_body<D, GRAD, VISC, RG>(args[r], lb * 256 + threadIdx.x);
 }
 // (batch members are undecomposed: their tensors are never stored)
 template <int D, int SOLVER, bool VISC>

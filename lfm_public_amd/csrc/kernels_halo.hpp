@@ -22,12 +22,14 @@ struct HaloFields {
     T *q, *dudx, *dTdx, *tauMC, *sigmaU;
     int ntot, D, kind;  // kind: bit 0 SOLVARS, bit 1 VISCOUS
     int q_records;      // q stored as 16-byte cell records (the FP32 layout)
+    int grad_stride;    // dudx / dTdx in per-cell records of this stride (FP32), 0: SoA
 };
 
 // index of component m of cell i in field f
 template <typename T>
 __device__ __forceinline__ size_t halo_index(const HaloFields<T> &h, const T *f, int m, int i) {
     if (f == h.q && h.q_records) return m < 4 ? 4 * (size_t)i + m : 4 * (size_t)h.ntot + i;
+    if ((f == h.dudx || f == h.dTdx) && h.grad_stride) return (size_t)i * h.grad_stride + m;
     return (size_t)m * h.ntot + i;
 }
 

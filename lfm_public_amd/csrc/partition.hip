@@ -41,7 +41,7 @@ static constexpr int kBlock = 256;
 static inline int nblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 template <typename T>
-static HaloFields<T> fields_of(T *q, T *dudx, T *dTdx, T *tau, T *sU, int ntot, int D, int kind) {
+static HaloFields<T> fields_of(T *q, T *dudx, T *dTdx, T *tau, T *sU, int ntot, int D, int kind, int grad_stride) {
     HaloFields<T> h;
     h.q = q;
     h.dudx = dudx;
@@ -52,6 +52,7 @@ static HaloFields<T> fields_of(T *q, T *dudx, T *dTdx, T *tau, T *sU, int ntot, 
     h.D = D;
     h.kind = kind;
     h.q_records = sizeof(T) == 4;  // the FP32 state is stored as cell records
+    h.grad_stride = grad_stride;
     return h;
 }
 
@@ -64,12 +65,12 @@ int Solver::halo_pack(int ni, int kind, void *buf) {
     if (fp32) {
         KArgs32 &a = *k32;
         hipLaunchKernelGGL(halo_pack_kernel<float>, dim3(nblocks(n)), dim3(kBlock), 0, stream,
-                           fields_of<float>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind), hd.send_cells,
+                           fields_of<float>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind, a.gstride), hd.send_cells,
                            hd.n_send, rec, (float *)buf);
     } else {
         KArgs &a = k64;
         hipLaunchKernelGGL(halo_pack_kernel<double>, dim3(nblocks(n)), dim3(kBlock), 0, stream,
-                           fields_of<double>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind), hd.send_cells,
+                           fields_of<double>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind, 0), hd.send_cells,
                            hd.n_send, rec, (double *)buf);
     }
     return LFM_OK;
@@ -84,12 +85,12 @@ int Solver::halo_unpack(int ni, int kind, const void *buf) {
     if (fp32) {
         KArgs32 &a = *k32;
         hipLaunchKernelGGL(halo_unpack_kernel<float>, dim3(nblocks(n)), dim3(kBlock), 0, stream,
-                           fields_of<float>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind), hd.recv_ghost,
+                           fields_of<float>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind, a.gstride), hd.recv_ghost,
                            hd.recv_src, hd.n_recv, rec, (const float *)buf);
     } else {
         KArgs &a = k64;
         hipLaunchKernelGGL(halo_unpack_kernel<double>, dim3(nblocks(n)), dim3(kBlock), 0, stream,
-                           fields_of<double>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind), hd.recv_ghost,
+                           fields_of<double>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind, 0), hd.recv_ghost,
                            hd.recv_src, hd.n_recv, rec, (const double *)buf);
     }
     return LFM_OK;

@@ -126,8 +126,10 @@ int create_fp32p(Solver *s) {
     }
     rc |= up(s, &a.dq, (float *)nullptr, (size_t)(D + 2) * nc);
     rc |= up(s, &a.res_cell, (float *)nullptr, (size_t)(D + 2) * nc);
-    rc |= up(s, &a.dudx, (float *)nullptr, (size_t)D * D * ntot);
-    rc |= up(s, &a.dTdx, (float *)nullptr, (size_t)D * ntot);
+    // gradient records (GradRec32): dudx, dTdx and (M2AUSM) rho / p gradients
+    const bool rg = s->prm.solver == 2;
+    a.gstride = grad_record_stride(D, rg);
+    rc |= up(s, &a.dudx, (float *)nullptr, (size_t)a.gstride * ntot);
     // tauMC / sigmaU are kept only for the halo payload of a partition (the
     // face kernel otherwise recomputes them from dudx); U_grad shares dudx's
     // storage (equal bit for bit) except where halo ghosts must read zero
@@ -137,9 +139,11 @@ int create_fp32p(Solver *s) {
         rc |= up(s, &a.tauMC, (float *)nullptr, (size_t)D * D * ntot);
         rc |= up(s, &a.sigmaU, (float *)nullptr, (size_t)D * ntot);
     }
+    if (rc) return LFM_EDEVICE;
+    a.dTdx = a.dudx + D * D;
     if (s->prm.solver == 2) {
-        rc |= up(s, &a.rho_grad, (float *)nullptr, (size_t)D * ntot);
-        rc |= up(s, &a.p_grad, (float *)nullptr, (size_t)D * ntot);
+        a.rho_grad = a.dudx + D * D + D;
+        a.p_grad = a.dudx + D * D + 2 * D;
         if (part || !s->prm.minmod)
             rc |= up(s, &a.U_grad, (float *)nullptr, (size_t)D * D * ntot);
         else
@@ -171,9 +175,11 @@ template <int D, bool V>
 static void launch_gradvis32_v(Solver *s) {
     const int nb = nblocks(s->L.nc);
     if (s->prm.solver == 2 && s->prm.minmod)
-        hipLaunchKernelGGL((grad_vis32<D, 2, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+        hipLaunchKernelGGL((grad_vis32<D, 2, V, true>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+    else if (s->prm.solver == 2)
+        hipLaunchKernelGGL((grad_vis32<D, 0, V, true>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
     else
-        hipLaunchKernelGGL((grad_vis32<D, 0, V>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+        hipLaunchKernelGGL((grad_vis32<D, 0, V, false>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
 }
 
 template <int D>
@@ -286,6 +292,18 @@ int get_field32(Solver *s, int which, double *out) {
         return LFM_OK;
     }
     if (!ptr) return fail(LFM_EINVAL, "field not allocated for this solver");
+    const bool in_record = ptr != a.q && ptr != a.dq && ptr != a.res_cell && ptr != a.tauMC && ptr != a.sigmaU &&
+                           (ptr != a.U_grad || a.U_grad == a.dudx);
+    if (in_record) {
+        // component k of cell ci at ptr[ci * gstride + k]
+        std::vector<float> recs((size_t)a.gstride * nc);
+        HIP_TRY(hipMemcpy(recs.data(), a.dudx, recs.size() * sizeof(float), hipMemcpyDeviceToHost));
+        const size_t off = (size_t)(ptr - a.dudx);
+        for (int k = 0; k < ncomp; k++)
+            for (size_t ci = 0; ci < nc; ci++)
+                out[k * nc + L.orig[ci]] = (double)recs[ci * a.gstride + off + k];
+        return LFM_OK;
+    }
     double qinf[5] = {a.rho_inf_d, a.m_inf_d[0], a.m_inf_d[1], a.m_inf_d[2], 0.0};
     if (which == LFM_FIELD_Q) {
         // cell records: components 0-3 interleaved, component 4 after them
