@@ -887,15 +887,7 @@ __device__ __forceinline__ void cfl_dt_cells32_body(const KArgs32 &a, double par
 #pragma unroll
         for (int k = 0; k < D; k++) m[k] = a.m_inf_d[k] + (double)a.q[q32_index(k + 1, c, a.ntot)];
         double sum = 0.0;
-        for (int e = a.s_off[c]; e < a.s_off[c + 1]; e++) {
-            const int ent = a.s_ent[e];
-            const int f = ent >= 0 ? ent : ~ent;
-            double S[D];
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                const double s = (double)a.S[(size_t)k * a.nf + f];
-                S[k] = ent >= 0 ? s : -s;
-            }
+        auto add = [&](const double (&S)[D]) {
             double fl = 0.0;
             if (MODE == 0) {
                 double sM = 0.0;
@@ -909,6 +901,34 @@ __device__ __forceinline__ void cfl_dt_cells32_body(const KArgs32 &a, double par
                 for (int k = 0; k < D; k++) fl += m[k] * S[k];
             }
             sum += fabs(fl);
+        };
+        // slot-ordered sum; the first slots' area vectors fetched together
+        const int e0 = a.s_off[c], ne = a.s_off[c + 1] - e0;
+        constexpr int U = GatherUnroll<D>::value;
+        double Sj[U][D];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const int ent = a.s_ent[e0 + (j < ne ? j : ne - 1)];
+            const int f = ent >= 0 ? ent : ~ent;
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                const double sv = (double)a.S[(size_t)k * a.nf + f];
+                Sj[j][k] = ent >= 0 ? sv : -sv;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++)
+            if (j < ne) add(Sj[j]);
+        for (int e = e0 + U; e < e0 + ne; e++) {
+            const int ent = a.s_ent[e];
+            const int f = ent >= 0 ? ent : ~ent;
+            double S[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                const double sv = (double)a.S[(size_t)k * a.nf + f];
+                S[k] = ent >= 0 ? sv : -sv;
+            }
+            add(S);
         }
         const double vol = a.vol_d[c];
         if (MODE == 0) {

@@ -848,12 +848,7 @@ __global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
         double q[D + 2];
         load_q<D>(a, c, q);
         double sum = 0.0;
-        for (int e = a.s_off[c]; e < a.s_off[c + 1]; e++) {
-            const int ent = a.s_ent[e];
-            const int f = ent >= 0 ? ent : ~ent;  // slot lists carry no ghost flag
-            double S[D];
-#pragma unroll
-            for (int k = 0; k < D; k++) S[k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
+        auto add = [&](const double (&S)[D]) {
             if (MODE == 0) {
                 double sM = 0.0;
 #pragma unroll
@@ -869,6 +864,29 @@ __global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
                 for (int k = 0; k < D; k++) fl += q[k + 1] * S[k];
                 sum += fabs(fl);
             }
+        };
+        // slot-ordered sum; the first slots' area vectors fetched together
+        // (slot lists carry no ghost flag)
+        const int e0 = a.s_off[c], ne = a.s_off[c + 1] - e0;
+        constexpr int U = D == 2 ? 4 : 6;
+        double Sj[U][D];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const int ent = a.s_ent[e0 + (j < ne ? j : ne - 1)];
+            const int f = ent >= 0 ? ent : ~ent;
+#pragma unroll
+            for (int k = 0; k < D; k++) Sj[j][k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++)
+            if (j < ne) add(Sj[j]);
+        for (int e = e0 + U; e < e0 + ne; e++) {
+            const int ent = a.s_ent[e];
+            const int f = ent >= 0 ? ent : ~ent;
+            double S[D];
+#pragma unroll
+            for (int k = 0; k < D; k++) S[k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
+            add(S);
         }
         if (MODE == 0) {
             const double vol = 1.0 / a.vinv[c];
