@@ -37,29 +37,31 @@ void Solver::zero_state_for_step() {
 
 namespace {
 
-// (device pointer, components, stride) of a field in the FP64 layout
+// A field in the FP64 layout: component k of cell ci at ptr[k * cs + ci * es],
+// or, for the state (q), at ptr[q64_index(k, ci, ntot)]
 struct FieldRef {
     double *ptr;
     int ncomp;
-    size_t stride;
+    size_t cs, es;
+    bool state;
 };
 
 FieldRef field_ref64(Solver *s, int which) {
     const int D = s->L.D;
-    const size_t nt = (size_t)s->L.nc + s->L.ng, nc = s->L.nc;
+    const size_t nt = (size_t)s->L.nc + s->L.ng, nc = s->L.nc, gs = (size_t)s->k64.gstride;
     KArgs &a = s->k64;
     switch (which) {
-    case LFM_FIELD_Q: return {a.q, D + 2, nt};
-    case LFM_FIELD_DQ: return {a.dq, D + 2, nc};
-    case LFM_FIELD_DUDX: return {a.dudx, D * D, nt};
-    case LFM_FIELD_DTDX: return {a.dTdx, D, nt};
-    case LFM_FIELD_TAUMC: return {a.tauMC, D * D, nt};
-    case LFM_FIELD_SIGMAU: return {a.sigmaU, D, nt};
-    case LFM_FIELD_RHOGRAD: return {a.rho_grad, D, nt};
-    case LFM_FIELD_PGRAD: return {a.p_grad, D, nt};
-    case LFM_FIELD_UGRAD: return {a.U_grad, D * D, nt};
-    case LFM_FIELD_RES: return {a.res_cell, D + 2, nc};
-    default: return {nullptr, 0, 0};
+    case LFM_FIELD_Q: return {a.q, D + 2, 0, 0, true};
+    case LFM_FIELD_DQ: return {a.dq, D + 2, nc, 1, false};
+    case LFM_FIELD_DUDX: return {a.dudx, D * D, 1, gs, false};
+    case LFM_FIELD_DTDX: return {a.dTdx, D, 1, gs, false};
+    case LFM_FIELD_TAUMC: return {a.tauMC, D * D, nt, 1, false};
+    case LFM_FIELD_SIGMAU: return {a.sigmaU, D, nt, 1, false};
+    case LFM_FIELD_RHOGRAD: return {a.rho_grad, D, 1, gs, false};
+    case LFM_FIELD_PGRAD: return {a.p_grad, D, 1, gs, false};
+    case LFM_FIELD_UGRAD: return {a.U_grad, D * D, 1, gs, false};
+    case LFM_FIELD_RES: return {a.res_cell, D + 2, nc, 1, false};
+    default: return {nullptr, 0, 0, 0, false};
     }
 }
 
@@ -111,11 +113,25 @@ int Solver::get_field(int which, double *out) {
         return LFM_OK;
     }
     if (!r.ptr) return fail(LFM_EINVAL, "unknown or unallocated field");
-    std::vector<double> tmp(nc);
-    for (int k = 0; k < r.ncomp; k++) {
-        HIP_TRY(hipMemcpy(tmp.data(), r.ptr + k * r.stride, nc * sizeof(double), hipMemcpyDeviceToHost));
-        double *o = out + k * nc;
-        for (size_t ci = 0; ci < nc; ci++) o[L.orig[ci]] = tmp[ci];
+    const size_t nt = (size_t)L.nc + L.ng;
+    if (r.state) {
+        std::vector<double> tmp((size_t)(D + 2) * nt);
+        HIP_TRY(hipMemcpy(tmp.data(), r.ptr, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < r.ncomp; k++)
+            for (size_t ci = 0; ci < nc; ci++) out[k * nc + L.orig[ci]] = tmp[q64_index(k, ci, nt)];
+    } else if (r.es == 1) {
+        std::vector<double> tmp(nc);
+        for (int k = 0; k < r.ncomp; k++) {
+            HIP_TRY(hipMemcpy(tmp.data(), r.ptr + k * r.cs, nc * sizeof(double), hipMemcpyDeviceToHost));
+            double *o = out + k * nc;
+            for (size_t ci = 0; ci < nc; ci++) o[L.orig[ci]] = tmp[ci];
+        }
+    } else {
+        // per-cell records: one copy of the cells' records from the field's slot
+        std::vector<double> tmp((nc - 1) * r.es + r.ncomp);
+        HIP_TRY(hipMemcpy(tmp.data(), r.ptr, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < r.ncomp; k++)
+            for (size_t ci = 0; ci < nc; ci++) out[k * nc + L.orig[ci]] = tmp[ci * r.es + k * r.cs];
     }
     return LFM_OK;
 }
@@ -148,11 +164,21 @@ int Solver::set_field(int which, const double *in) {
     }
     FieldRef r = field_ref64(this, which);
     if (!r.ptr || (which != LFM_FIELD_Q && which != LFM_FIELD_DQ)) return fail(LFM_EINVAL, "field not settable");
+    if (r.state) {
+        // the cells' records (ghost slots kept)
+        const size_t nt = (size_t)L.nc + L.ng;
+        std::vector<double> tmp((size_t)(L.D + 2) * nt);
+        HIP_TRY(hipMemcpy(tmp.data(), r.ptr, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < r.ncomp; k++)
+            for (size_t ci = 0; ci < nc; ci++) tmp[q64_index(k, ci, nt)] = in[k * nc + L.orig[ci]];
+        HIP_TRY(hipMemcpy(r.ptr, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+        return LFM_OK;
+    }
     std::vector<double> tmp(nc);
     for (int k = 0; k < r.ncomp; k++) {
         const double *src = in + k * nc;
         for (size_t ci = 0; ci < nc; ci++) tmp[ci] = src[L.orig[ci]];
-        HIP_TRY(hipMemcpy(r.ptr + k * r.stride, tmp.data(), nc * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(r.ptr + k * r.cs, tmp.data(), nc * sizeof(double), hipMemcpyHostToDevice));
     }
     return LFM_OK;
 }

@@ -30,12 +30,17 @@ struct KArgs {
     const int32_t *ghost_cell, *ghost_role;
     const double *vinv, *sigma;
     const double *smag;             // [nc] Smagorinsky constants, nullptr when laminar
-    double *q;                      // [(D+2)][ntot]
+    double *q;                      // cell records (q64_index), ghosts included
     double *dq;                     // [(D+2)][nc]
     double *res_cell;               // [(D+2)][nc]
-    double *dudx, *dTdx, *tauMC, *sigmaU;  // [D*D|D][ntot], ghost part zero
-    double *rho_grad, *p_grad, *U_grad;    // [D|D*D][ntot], ghost part zero
-    double *frhs;                   // [(D+2)][nf]
+    // per-cell gradient records (GradRec64; ghosts' stay zero): dudx points
+    // at the records, dTdx / rho_grad / p_grad / U_grad into them (component
+    // stride 1, cell stride gstride); tauMC / sigmaU (partitions only) are
+    // [component][ntot]
+    double *dudx, *dTdx, *tauMC, *sigmaU;
+    double *rho_grad, *p_grad, *U_grad;
+    int gstride;
+    double *frhs;                   // face records: [nf][4], then (3D) [nf]
     const double *dt;               // device scalar (deltaT)
     double *partial;                // reduction partials
     // gas
@@ -75,10 +80,71 @@ __device__ __forceinline__ double dotD(const double *x, const double *y) {
     return s;
 }
 
+// State layout: components 0-3 of a cell (or ghost) i as one 32-byte
+// record, q[4 i + k], then component 4 (the 3D energy) as q[4 ntot + i]: a
+// gathered neighbour state is two 16-byte loads (three in 3D).
+__host__ __device__ __forceinline__ size_t q64_index(int k, size_t i, size_t ntot) {
+    return k < 4 ? 4 * i + (size_t)k : 4 * ntot + i;
+}
+
 template <int D>
 __device__ __forceinline__ void load_q(const KArgs &a, int i, double *q) {
+    const double2 *p = reinterpret_cast<const double2 *>(a.q + 4 * (size_t)i);
+    const double2 v0 = p[0], v1 = p[1];
+    q[0] = v0.x;
+    q[1] = v0.y;
+    q[2] = v1.x;
+    q[3] = v1.y;
+    if (D == 3) q[4] = a.q[(size_t)4 * a.ntot + i];
+}
+
+template <int D>
+__device__ __forceinline__ void store_q(double *qv, size_t ntot, int i, const double *q) {
+    double2 *p = reinterpret_cast<double2 *>(qv + 4 * (size_t)i);
+    p[0] = make_double2(q[0], q[1]);
+    p[1] = make_double2(q[2], q[3]);
+    if (D == 3) qv[4 * ntot + i] = q[4];
+}
+
+// One cell's gradients as one record, padded to whole 16-byte words: dudx
+// (D*D), dTdx (D), then with RG (the M2AUSM scheme) the rho and p gradients
+// (D each) and U_grad (D*D).
+template <int D, bool RG>
+struct GradRec64 {
+    static constexpr int kDT = D * D;
+    static constexpr int kRG = kDT + D;
+    static constexpr int kPG = kRG + D;
+    static constexpr int kUG = kPG + D;
+    static constexpr int kN = RG ? kUG + D * D : kRG;
+    static constexpr int kStride = (kN + 1) / 2 * 2;
+};
+inline int grad_record_stride64(int D, bool rg) {
+    const int n = D * D + D + (rg ? 2 * D + D * D : 0);
+    return (n + 1) / 2 * 2;
+}
+
+// face flux record: components 0-3 as 32 bytes per face, component 4 after
+template <int D>
+__device__ __forceinline__ void load_frhs(const KArgs &a, int f, double (&r)[D + 2]) {
+    const double2 *p = reinterpret_cast<const double2 *>(a.frhs + 4 * (size_t)f);
+    const double2 v0 = p[0], v1 = p[1];
+    r[0] = v0.x;
+    r[1] = v0.y;
+    r[2] = v1.x;
+    r[3] = v1.y;
+    if (D == 3) r[4] = a.frhs[(size_t)4 * a.nf + f];
+}
+
+template <int D, bool RG>
+__device__ __forceinline__ void load_grec(const KArgs &a, int c, double (&r)[GradRec64<D, RG>::kStride]) {
+    using R = GradRec64<D, RG>;
+    const double2 *p = reinterpret_cast<const double2 *>(a.dudx + (size_t)c * R::kStride);
 #pragma unroll
-    for (int k = 0; k < D + 2; k++) q[k] = a.q[(size_t)k * a.ntot + i];
+    for (int k = 0; k < R::kStride / 2; k++) {
+        const double2 v = p[k];
+        r[2 * k] = v.x;
+        r[2 * k + 1] = v.y;
+    }
 }
 
 __device__ __forceinline__ double sgn(double x) { return x < 0.0 ? -1.0 : 1.0; }
@@ -185,8 +251,7 @@ __global__ void __launch_bounds__(256) bc_fill(KArgs a) {
     load_q<D>(a, a.ghost_cell[g], qi);
     if (!ghost_state<D>(a, a.ghost_role[g], qi, qg)) return;
     const int gi = a.nc + g;
-#pragma unroll
-    for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + gi] = qg[k];
+    store_q<D>(a.q, a.ntot, gi, qg);
 }
 
 // ---------------------------------------------------------------------------
@@ -261,10 +326,16 @@ __device__ __forceinline__ void vis_tensors(const KArgs &a, const double (&dudx)
     for (int i = 0; i < D; i++) sU[i] = dotD<D>(U, tau[i]);
 }
 
-template <int D, int GRAD, bool VISC>
+template <int D, int GRAD, bool VISC, bool RG>
 __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
     const int c = LFM_TID(blockIdx.x);
-    if (c >= a.nc) return;
+    using R = GradRec64<D, RG>;
+    // every thread reaches the record write-out below (LDS staging)
+    const bool valid = c < a.nc;
+    double rec[R::kStride];
+#pragma unroll
+    for (int k = 0; k < R::kStride; k++) rec[k] = 0.0;
+    if (valid) {
     const double vinv = a.vinv[c];
     double dudx[D][D], dTdx[D];
     double rg[D], pg[D], ug[D][D];
@@ -377,17 +448,38 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
 #pragma unroll
     for (int i = 0; i < D; i++) {
 #pragma unroll
-        for (int j = 0; j < D; j++) a.dudx[(i * D + j) * nt + c] = dudx[i][j];
-        if (VISC) a.dTdx[i * nt + c] = dTdx[i];
-    }
-    if (GRAD == 2) {
+        for (int j = 0; j < D; j++) rec[i * D + j] = dudx[i][j];
+        rec[R::kDT + i] = VISC ? dTdx[i] : 0.0;
+        if (RG && GRAD == 2) {
+            rec[R::kRG + i] = rg[i];
+            rec[R::kPG + i] = pg[i];
 #pragma unroll
-        for (int i = 0; i < D; i++) {
-            a.rho_grad[i * nt + c] = rg[i];
-            a.p_grad[i * nt + c] = pg[i];
-#pragma unroll
-            for (int j = 0; j < D; j++) a.U_grad[(i * D + j) * nt + c] = ug[i][j];
+            for (int j = 0; j < D; j++) rec[R::kUG + i * D + j] = ug[i][j];
         }
+    }
+    if (!VISC) {
+        // dTdx not formed: the formed components only (the record keeps
+        // whatever a full calc_VIS left in the dTdx slots)
+        double *out = a.dudx + (size_t)c * R::kStride;
+#pragma unroll
+        for (int k = 0; k < R::kN; k++)
+            if (k < R::kDT || k >= R::kRG) out[k] = rec[k];
+    }
+    }
+    if (VISC) {
+        // the block's records are contiguous from its first cell: staged in
+        // LDS, then written as consecutive 16-byte words (without minmod the
+        // rho / p / U gradient slots are written zero, as they stay)
+        constexpr int W = R::kStride / 2;
+        __shared__ double2 st[256 * W];
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < W; k++) st[t * W + k] = make_double2(rec[2 * k], rec[2 * k + 1]);
+        __syncthreads();
+        const int c0 = c - t;
+        const int nrec = a.nc - c0 < (int)blockDim.x ? a.nc - c0 : (int)blockDim.x;
+        double2 *dst = reinterpret_cast<double2 *>(a.dudx + (size_t)c0 * R::kStride);
+        for (int j = t; j < nrec * W; j += blockDim.x) dst[j] = st[j];
     }
 }
 
@@ -416,6 +508,12 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
     }
     load_q<D>(a, P, qP);
     load_q<D>(a, N, qN);
+    using R = GradRec64<D, SOLVER == 2>;
+    double gr[2][R::kStride];  // gradient records of P and N
+    if (VISC || SOLVER == 2) {
+        load_grec<D, SOLVER == 2>(a, P, gr[0]);
+        load_grec<D, SOLVER == 2>(a, N, gr[1]);
+    }
     const double cri = 1.0 / qP[0], ari = 1.0 / qN[0];
     double cU[D], aU[D];
 #pragma unroll
@@ -508,8 +606,8 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
             for (int i = 0; i < D; i++)
 #pragma unroll
                 for (int j = 0; j < D; j++) {
-                    cdu[i][j] = a.dudx[(i * D + j) * nt + P];
-                    ndu[i][j] = a.dudx[(i * D + j) * nt + N];
+                    cdu[i][j] = gr[0][i * D + j];
+                    ndu[i][j] = gr[1][i * D + j];
                 }
 #pragma unroll
             for (int p = 0; p < D; p++) {
@@ -527,8 +625,8 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
                 double gc[D], gn[D];
 #pragma unroll
                 for (int k = 0; k < D; k++) {
-                    gc[k] = a.U_grad[(p * D + k) * nt + P];
-                    gn[k] = a.U_grad[(p * D + k) * nt + N];
+                    gc[k] = gr[0][R::kUG + p * D + k];
+                    gn[k] = gr[1][R::kUG + p * D + k];
                 }
                 const double uPos = interp_minmod<D>(cu, nu, gc, d, w, 1.0);
                 const double uNeg = interp_minmod<D>(cu, nu, gn, d, w, -1.0);
@@ -549,10 +647,10 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
             double gc[D], gn[D], hc[D], hn[D];
 #pragma unroll
             for (int k = 0; k < D; k++) {
-                gc[k] = a.rho_grad[k * nt + P];
-                gn[k] = a.rho_grad[k * nt + N];
-                hc[k] = a.p_grad[k * nt + P];
-                hn[k] = a.p_grad[k * nt + N];
+                gc[k] = gr[0][R::kRG + k];
+                gn[k] = gr[1][R::kRG + k];
+                hc[k] = gr[0][R::kPG + k];
+                hn[k] = gr[1][R::kPG + k];
             }
             const double rhoP = interp_minmod<D>(qP[0], qN[0], gc, d, w, 1.0);
             const double rhoN = interp_minmod<D>(qP[0], qN[0], gn, d, w, -1.0);
@@ -603,11 +701,11 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
         for (int i = 0; i < D; i++) {
 #pragma unroll
             for (int j = 0; j < D; j++) {
-                duP[i][j] = a.dudx[(i * D + j) * nt + P];
-                duN[i][j] = a.dudx[(i * D + j) * nt + N];
+                duP[i][j] = gr[0][i * D + j];
+                duN[i][j] = gr[1][i * D + j];
                 du[i][j] = INTERP_LINEAR(w, duP[i][j], duN[i][j]);
             }
-            dT[i] = INTERP_LINEAR(w, a.dTdx[i * nt + P], a.dTdx[i * nt + N]);
+            dT[i] = INTERP_LINEAR(w, gr[0][R::kDT + i], gr[1][R::kDT + i]);
         }
     }
 #pragma unroll
@@ -681,8 +779,10 @@ __global__ void __launch_bounds__(256) face_flux(KArgs a) {
     const double lapT = kcond * (dm * (aT - cT) * dmi + dotD<D>(K, dT));
     rhs[D + 1] += dotD<D>(sU, S) + lapT;
     }
-#pragma unroll
-    for (int k = 0; k < D + 2; k++) a.frhs[(size_t)k * a.nf + f] = rhs[k];
+    double2 *fo = reinterpret_cast<double2 *>(a.frhs + 4 * (size_t)f);
+    fo[0] = make_double2(rhs[0], rhs[1]);
+    fo[1] = make_double2(rhs[2], rhs[3]);
+    if (D == 3) a.frhs[(size_t)4 * a.nf + f] = rhs[D + 1];
 }
 
 // tauMC and sigmaU of every cell from the stored dudx (field read-back of a
@@ -695,7 +795,7 @@ __global__ void __launch_bounds__(256) vis_tensor_field(KArgs a, double *tmc_out
 #pragma unroll
     for (int i = 0; i < D; i++)
 #pragma unroll
-        for (int j = 0; j < D; j++) du[i][j] = a.dudx[(size_t)(i * D + j) * a.ntot + c];
+        for (int j = 0; j < D; j++) du[i][j] = a.dudx[(size_t)c * a.gstride + i * D + j];
     load_q<D>(a, c, q);
     vis_tensors<D>(a, du, q, a.smag ? a.smag + c : nullptr, tmc, sU);
 #pragma unroll
@@ -754,8 +854,7 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
         for (int j = 0; j < U; j++) {
             ent[j] = a.g_ent[e0 + (j < ne ? j : ne - 1)];
             const int f = ent[j] >= 0 ? (ent[j] & kFaceMask) : ~ent[j];
-#pragma unroll
-            for (int k = 0; k < D + 2; k++) fr[j][k] = a.frhs[(size_t)k * a.nf + f];
+            load_frhs<D>(a, f, fr[j]);
         }
 #pragma unroll
         for (int j = 0; j < U; j++)
@@ -764,8 +863,7 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
             const int en = a.g_ent[e];
             const int f = en >= 0 ? (en & kFaceMask) : ~en;
             double f1[D + 2];
-#pragma unroll
-            for (int k = 0; k < D + 2; k++) f1[k] = a.frhs[(size_t)k * a.nf + f];
+            load_frhs<D>(a, f, f1);
             add(en, f1);
         }
         double q[D + 2];
@@ -778,16 +876,15 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
 #pragma unroll
         for (int k = 0; k < D + 2; k++) {
             q[k] += B * dq[k];
-            a.q[(size_t)k * a.ntot + c] = q[k];
             a.dq[(size_t)k * a.nc + c] = dq[k];
         }
+        store_q<D>(a.q, a.ntot, c, q);
         // refresh this cell's boundary ghosts for the next stage
         for (int i = 0; i < ngf; i++) {
             const int g = a.fN[gface[i]] - a.nc;
             double qg[D + 2];
             if (!ghost_state<D>(a, a.ghost_role[g], q, qg)) continue;
-#pragma unroll
-            for (int k = 0; k < D + 2; k++) a.q[(size_t)k * a.ntot + a.nc + g] = qg[k];
+            store_q<D>(a.q, a.ntot, a.nc + g, qg);
         }
         if (RES) {
 #pragma unroll
@@ -953,7 +1050,7 @@ __global__ void __launch_bounds__(256) integrals64(KArgs a) {
     __shared__ double red[256 / 64][D + 2];
     double v[D + 2];
 #pragma unroll
-    for (int k = 0; k < D + 2; k++) v[k] = c < a.nc ? a.q[(size_t)k * a.ntot + c] / a.vinv[c] : 0.0;
+    for (int k = 0; k < D + 2; k++) v[k] = c < a.nc ? a.q[q64_index(k, c, a.ntot)] / a.vinv[c] : 0.0;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < D + 2; k++) {
@@ -983,7 +1080,7 @@ __global__ void __launch_bounds__(256) count_nonfinite(const double *q, int nc, 
     if (c >= nc) return;
     int bad = 0;
 #pragma unroll
-    for (int k = 0; k < D + 2; k++) bad += !isfinite(q[(size_t)k * ntot + c]);
+    for (int k = 0; k < D + 2; k++) bad += !isfinite(q[q64_index(k, c, ntot)]);
     if (bad) atomicAdd(cnt, (unsigned long long)bad);
 }
 

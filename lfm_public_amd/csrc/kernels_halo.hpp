@@ -5,10 +5,9 @@
 //   SOLVARS: q[0..D+1]
 //   VISCOUS: dudx[i][j] (row-major), dTdx[i], tauMC[i][j], sigmaU[i]
 //   PACKED : SOLVARS then VISCOUS
-// Device fields are SoA [component][ntot] (component m of a record reads
-// field[m * ntot + cell]), except the FP32 state, stored as cell records:
-// one thread per (record, component) keeps the buffer writes (pack) and
-// reads (unpack) contiguous.
+// The state and the gradients are stored as per-cell records, tauMC and
+// sigmaU as [component][ntot]: one thread per (record, component) keeps the
+// buffer writes (pack) and reads (unpack) contiguous.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,8 +20,8 @@ template <typename T>
 struct HaloFields {
     T *q, *dudx, *dTdx, *tauMC, *sigmaU;
     int ntot, D, kind;  // kind: bit 0 SOLVARS, bit 1 VISCOUS
-    int q_records;      // q stored as 16-byte cell records (the FP32 layout)
-    int grad_stride;    // dudx / dTdx in per-cell records of this stride (FP32), 0: SoA
+    int q_records;      // q stored as cell records (components 0-3, then 4)
+    int grad_stride;    // dudx / dTdx in per-cell records of this stride, 0: SoA
 };
 
 // index of component m of cell i in field f

@@ -51,7 +51,7 @@ static HaloFields<T> fields_of(T *q, T *dudx, T *dTdx, T *tau, T *sU, int ntot, 
     h.ntot = ntot;
     h.D = D;
     h.kind = kind;
-    h.q_records = sizeof(T) == 4;  // the FP32 state is stored as cell records
+    h.q_records = 1;  // the state is stored as cell records
     h.grad_stride = grad_stride;
     return h;
 }
@@ -70,7 +70,7 @@ int Solver::halo_pack(int ni, int kind, void *buf) {
     } else {
         KArgs &a = k64;
         hipLaunchKernelGGL(halo_pack_kernel<double>, dim3(nblocks(n)), dim3(kBlock), 0, stream,
-                           fields_of<double>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind, 0), hd.send_cells,
+                           fields_of<double>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind, a.gstride), hd.send_cells,
                            hd.n_send, rec, (double *)buf);
     }
     return LFM_OK;
@@ -90,7 +90,7 @@ int Solver::halo_unpack(int ni, int kind, const void *buf) {
     } else {
         KArgs &a = k64;
         hipLaunchKernelGGL(halo_unpack_kernel<double>, dim3(nblocks(n)), dim3(kBlock), 0, stream,
-                           fields_of<double>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind, 0), hd.recv_ghost,
+                           fields_of<double>(a.q, a.dudx, a.dTdx, a.tauMC, a.sigmaU, ntot, L.D, kind, a.gstride), hd.recv_ghost,
                            hd.recv_src, hd.n_recv, rec, (const double *)buf);
     }
     return LFM_OK;

@@ -44,9 +44,8 @@ namespace lfm {
 static constexpr int kPostBlock = 256;
 static constexpr int kHistRows = 1024;  // force rows kept on the device before a drain
 
-// State as stored: value = base[k] + (double) q(k, i) (base 0 for the FP64
-// path, the free stream for the FP32 deviation path, whose q is stored as
-// cell records).
+// State as stored (cell records): value = base[k] + (double) q(k, i) (base 0
+// for the FP64 path, the free stream for the FP32 deviation path).
 template <typename T>
 struct QView {
     const T *q;
@@ -56,10 +55,8 @@ struct QView {
 
 template <typename T>
 __device__ __forceinline__ double qv(const QView<T> &v, int k, int i) {
-    if constexpr (sizeof(T) == 4)  // FP32 cell records
-        return v.base[k] + (double)v.q[q32_index(k, (size_t)i, (size_t)v.ntot)];
-    else
-        return v.base[k] + (double)v.q[(size_t)k * v.ntot + i];
+    // cell records (components 0-3, then 4) in either precision
+    return v.base[k] + (double)v.q[q64_index(k, (size_t)i, (size_t)v.ntot)];
 }
 
 template <int D, typename T>

@@ -110,23 +110,24 @@ static int create_fp64(Solver *s) {
     rc |= dev_zeros(s, &a.q, (size_t)(D + 2) * ntot);
     rc |= dev_zeros(s, &a.dq, (size_t)(D + 2) * nc);
     rc |= dev_zeros(s, &a.res_cell, (size_t)(D + 2) * nc);
-    rc |= dev_zeros(s, &a.dudx, (size_t)D * D * ntot);
-    rc |= dev_zeros(s, &a.dTdx, (size_t)D * ntot);
+    // gradient records (GradRec64): dudx, dTdx and (M2AUSM) the rho, p and U
+    // gradients; solver 2 without minmodExists reads their never-computed
+    // (zero) slots
+    const bool rg = s->prm.solver == 2;
+    a.gstride = grad_record_stride64(D, rg);
+    rc |= dev_zeros(s, &a.dudx, (size_t)a.gstride * ntot);
     // tauMC / sigmaU are kept only for the halo payload of a partition; the
     // face kernel otherwise recomputes them from dudx
     if (!L.halo.empty()) {
         rc |= dev_zeros(s, &a.tauMC, (size_t)D * D * ntot);
         rc |= dev_zeros(s, &a.sigmaU, (size_t)D * ntot);
     }
-    if (s->prm.solver == 2 && s->prm.minmod) {
-        rc |= dev_zeros(s, &a.rho_grad, (size_t)D * ntot);
-        rc |= dev_zeros(s, &a.p_grad, (size_t)D * ntot);
-        rc |= dev_zeros(s, &a.U_grad, (size_t)D * D * ntot);
-    } else if (s->prm.solver == 2) {
-        // solver 2 without minmodExists reads never-computed (zero) gradients
-        rc |= dev_zeros(s, &a.rho_grad, (size_t)D * ntot);
-        rc |= dev_zeros(s, &a.p_grad, (size_t)D * ntot);
-        rc |= dev_zeros(s, &a.U_grad, (size_t)D * D * ntot);
+    if (rc) return LFM_EDEVICE;
+    a.dTdx = a.dudx + D * D;
+    if (rg) {
+        a.rho_grad = a.dudx + D * D + D;
+        a.p_grad = a.dudx + D * D + 2 * D;
+        a.U_grad = a.dudx + D * D + 3 * D;
     }
     rc |= dev_zeros(s, &a.frhs, (size_t)(D + 2) * nf);
     const size_t npart = (size_t)nblocks((long)std::max(nc, nf)) * (D + 2) + 16;
@@ -149,10 +150,13 @@ static int create_fp64(Solver *s) {
     a.rho_inf = g.rho_inf;
     a.E_inf = g.E_inf;
     for (int k = 0; k < 3; k++) a.U_inf[k] = g.U_inf[k];
-    // initial state into q (cells part) and deltaT
-    for (int k = 0; k < D + 2; k++)
-        HIP_TRY(hipMemcpy(a.q + (size_t)k * ntot, L.q0.data() + (size_t)k * nc, nc * sizeof(double),
-                          hipMemcpyHostToDevice));
+    // initial state into q (cells part, as cell records) and deltaT
+    {
+        std::vector<double> q0((size_t)(D + 2) * ntot, 0.0);
+        for (int k = 0; k < D + 2; k++)
+            for (size_t c = 0; c < nc; c++) q0[q64_index(k, c, ntot)] = L.q0[(size_t)k * nc + c];
+        HIP_TRY(hipMemcpy(a.q, q0.data(), q0.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemcpy(s->d_scalar, &s->dt, sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }
@@ -169,9 +173,11 @@ template <int D, bool V>
 static void launch_gradvis64_v(Solver *s) {
     const int nb = nblocks(s->L.nc);
     if (s->prm.solver == 2 && s->prm.minmod)
-        hipLaunchKernelGGL((grad_vis<D, 2, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+        hipLaunchKernelGGL((grad_vis<D, 2, V, true>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+    else if (s->prm.solver == 2)
+        hipLaunchKernelGGL((grad_vis<D, 0, V, true>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
     else
-        hipLaunchKernelGGL((grad_vis<D, 0, V>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
+        hipLaunchKernelGGL((grad_vis<D, 0, V, false>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64);
 }
 
 // full: every tensor calc_VIS forms (the per-stage entry point); otherwise
