@@ -67,10 +67,14 @@ static void enqueue_step(Batch *b) {
         case 1: hipLaunchKernelGGL((face_flux32_b<D, 1, V>), dim3(b->nb_faces), blk, 0, b->stream, b->d_args, b->map_faces); break;
         default: hipLaunchKernelGGL((face_flux32_b<D, 2, V>), dim3(b->nb_faces), blk, 0, b->stream, b->d_args, b->map_faces); break;
         }
-        hipLaunchKernelGGL(cell_update32_b<D>, dim3(b->nb_cells), blk, 0, b->stream, b->d_args, b->map_cells,
-                           (float)b->A[rk], (float)b->B[rk], rk == 0 ? 1 : 0);
+        // the last stage's update also forms the step's compute_cfl partials
+        if (rk == b->nstages - 1)
+            hipLaunchKernelGGL((cell_update32_b<D, 1>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
+                               b->map_cells, (float)b->A[rk], (float)b->B[rk], rk == 0 ? 1 : 0);
+        else
+            hipLaunchKernelGGL((cell_update32_b<D, -1>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
+                               b->map_cells, (float)b->A[rk], (float)b->B[rk], rk == 0 ? 1 : 0);
     }
-    hipLaunchKernelGGL(cfl_cells32_b<D>, dim3(b->nb_cells), blk, 0, b->stream, b->d_args, b->map_cells);
     hipLaunchKernelGGL(cfl_final32_b, dim3((unsigned)b->members.size()), blk, 0, b->stream, b->d_args);
 }
 

@@ -773,10 +773,12 @@ struct FluxGlobal32 {
 };
 // ordered gather of one cell's face fluxes, sponge and RK update from the
 // state in a.q, written to qout (a.q itself for the in-place update) with
-// the cell's boundary ghosts refreshed there; res receives the residual
+// the cell's boundary ghosts refreshed there; res receives the residual, q
+// the updated state
 template <int D, bool RES, class FA>
 __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &fa, float A, float B, int zero_dq,
-                                                   const int c, float *qout, float (&res)[D + 2]) {
+                                                   const int c, float *qout, float (&res)[D + 2],
+                                                   float (&q)[D + 2]) {
     const float dt = (float)(*a.dt);
     const float vinv = a.vinv[c];
     float dq[D + 2];
@@ -817,7 +819,6 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
         fa.get(e, en >= 0 ? (en & kFaceMask) : ~en, f1);
         add(en, f1);
     }
-    float q[D + 2];
     load_q32<D>(a, c, q);
     const float sgm = a.sigma[c];
 #pragma unroll
@@ -836,17 +837,25 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
     }
 }
 
-// every thread of the block must call this (block-wide reduction when RES)
-template <int D, bool RES>
+template <int D, int MODE>
+__device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, const float (&qc)[D + 2], double param);
+
+// every thread of the block must call this (block-wide reductions when RES
+// or CFL).  CFL >= 0 (the last stage of a step): the step's compute_dt (0) /
+// compute_cfl (1) of the updated cells, per-block extrema into a.partial.
+template <int D, bool RES, int CFL>
 __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, float A, float B, int zero_dq, const int c,
-                                                   const int lblk) {
+                                                   const int lblk, double cfl_param) {
+    static_assert(!(RES && CFL >= 0), "the residual and the CFL share a.partial");
     __shared__ double red[256 / 64][D + 2];
     double rsq[D + 2];
+    double cv = CFL == 0 ? 100.0 : 0.0;
 #pragma unroll
     for (int k = 0; k < D + 2; k++) rsq[k] = 0.0;
     if (c < a.nc) {
-        float res[D + 2];
-        cell_update32_cell<D, RES>(a, FluxGlobal32<D>{a}, A, B, zero_dq, c, a.q, res);
+        float res[D + 2], q[D + 2];
+        cell_update32_cell<D, RES>(a, FluxGlobal32<D>{a}, A, B, zero_dq, c, a.q, res, q);
+        if constexpr (CFL >= 0) cv = cfl_cell32<D, CFL>(a, c, q, cfl_param);
         if (RES) {
 #pragma unroll
             for (int k = 0; k < D + 2; k++) {
@@ -873,86 +882,86 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, float A, fl
             }
         }
     }
+    if constexpr (CFL >= 0) block_extremum<CFL>(cv, a.partial + lblk);
 }
 
-// compute_dt / compute_cfl in FP64 from the FP32 deviations (every thread of
-// the block must call this)
+// compute_dt / compute_cfl (cfd_v0.cpp:2887-2942) of one cell in FP64 from
+// its FP32 deviations q: the sum over its faces in slot order, then dt_c
+// (MODE 0) or cfl_c (MODE 1), combined with the block's starting value so a
+// NaN never wins
+template <int D, int MODE>
+__device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, const float (&qc)[D + 2], double param) {
+    const double rho = a.rho_inf_d + (double)qc[0];
+    double m[D];
+#pragma unroll
+    for (int k = 0; k < D; k++) m[k] = a.m_inf_d[k] + (double)qc[k + 1];
+    double sum = 0.0;
+    auto add = [&](const double (&S)[D]) {
+        double fl = 0.0;
+        if (MODE == 0) {
+            double sM = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; k++) sM += S[k] * S[k];
+            sM = sqrt(sM);
+#pragma unroll
+            for (int k = 0; k < D; k++) fl += m[k] * S[k] / sM;
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; k++) fl += m[k] * S[k];
+        }
+        sum += fabs(fl);
+    };
+    // slot-ordered sum; the first slots' area vectors fetched together
+    const int e0 = a.s_off[c], ne = a.s_off[c + 1] - e0;
+    constexpr int U = GatherUnroll<D>::value;
+    double Sj[U][D];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+        const int ent = a.s_ent[e0 + (j < ne ? j : ne - 1)];
+        const int f = ent >= 0 ? ent : ~ent;
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const double sv = (double)a.S[(size_t)k * a.nf + f];
+            Sj[j][k] = ent >= 0 ? sv : -sv;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++)
+        if (j < ne) add(Sj[j]);
+    for (int e = e0 + U; e < e0 + ne; e++) {
+        const int ent = a.s_ent[e];
+        const int f = ent >= 0 ? ent : ~ent;
+        double S[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const double sv = (double)a.S[(size_t)k * a.nf + f];
+            S[k] = ent >= 0 ? sv : -sv;
+        }
+        add(S);
+    }
+    const double vol = a.vol_d[c];
+    double v = MODE == 0 ? 100.0 : 0.0;
+    if (MODE == 0) {
+        const double dtc = param * rho * 2.0 * vol / sum;
+        if (v > dtc) v = dtc;
+    } else {
+        const double cfl = sum * ((1.0 / rho) * (1.0 / vol) * param);
+        if (v < cfl) v = cfl;
+    }
+    return v;
+}
+
+// compute_dt / compute_cfl as their own launch (every thread of the block
+// must call this)
 template <int D, int MODE>
 __device__ __forceinline__ void cfl_dt_cells32_body(const KArgs32 &a, double param, const int c, const int lblk) {
-    __shared__ double sh[256];
     double v = MODE == 0 ? 100.0 : 0.0;
     if (c < a.nc) {
-        const double rho = a.rho_inf_d + (double)a.q[q32_index(0, c, a.ntot)];
-        double m[D];
-#pragma unroll
-        for (int k = 0; k < D; k++) m[k] = a.m_inf_d[k] + (double)a.q[q32_index(k + 1, c, a.ntot)];
-        double sum = 0.0;
-        auto add = [&](const double (&S)[D]) {
-            double fl = 0.0;
-            if (MODE == 0) {
-                double sM = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; k++) sM += S[k] * S[k];
-                sM = sqrt(sM);
-#pragma unroll
-                for (int k = 0; k < D; k++) fl += m[k] * S[k] / sM;
-            } else {
-#pragma unroll
-                for (int k = 0; k < D; k++) fl += m[k] * S[k];
-            }
-            sum += fabs(fl);
-        };
-        // slot-ordered sum; the first slots' area vectors fetched together
-        const int e0 = a.s_off[c], ne = a.s_off[c + 1] - e0;
-        constexpr int U = GatherUnroll<D>::value;
-        double Sj[U][D];
-#pragma unroll
-        for (int j = 0; j < U; j++) {
-            const int ent = a.s_ent[e0 + (j < ne ? j : ne - 1)];
-            const int f = ent >= 0 ? ent : ~ent;
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                const double sv = (double)a.S[(size_t)k * a.nf + f];
-                Sj[j][k] = ent >= 0 ? sv : -sv;
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < U; j++)
-            if (j < ne) add(Sj[j]);
-        for (int e = e0 + U; e < e0 + ne; e++) {
-            const int ent = a.s_ent[e];
-            const int f = ent >= 0 ? ent : ~ent;
-            double S[D];
-#pragma unroll
-            for (int k = 0; k < D; k++) {
-                const double sv = (double)a.S[(size_t)k * a.nf + f];
-                S[k] = ent >= 0 ? sv : -sv;
-            }
-            add(S);
-        }
-        const double vol = a.vol_d[c];
-        if (MODE == 0) {
-            const double dtc = param * rho * 2.0 * vol / sum;
-            if (v > dtc) v = dtc;
-        } else {
-            const double cfl = sum * ((1.0 / rho) * (1.0 / vol) * param);
-            if (v < cfl) v = cfl;
-        }
+        float q[D + 2];
+        load_q32<D>(a, c, q);
+        v = cfl_cell32<D, MODE>(a, c, q, param);
     }
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-            const double o = sh[threadIdx.x + st];
-            if (MODE == 0) {
-                if (sh[threadIdx.x] > o) sh[threadIdx.x] = o;
-            } else {
-                if (sh[threadIdx.x] < o) sh[threadIdx.x] = o;
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a.partial[lblk] = sh[0];
+    block_extremum<MODE>(v, a.partial + lblk);
 }
 
 // FP64 conservation integrals sum_c V_c q_c (per block partials)
@@ -1013,9 +1022,9 @@ template <int D, int SOLVER, bool VISC, bool STORED>
 __global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
     face_flux32_body<D, SOLVER, VISC, STORED>(a, LFM_TID(blockIdx.x));
 }
-template <int D, bool RES>
-__global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B, int zero_dq) {
-    cell_update32_body<D, RES>(a, A, B, zero_dq, LFM_TID(blockIdx.x), LFM_BLK());
+template <int D, bool RES, int CFL>
+__global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B, int zero_dq, double cfl_param) {
+    cell_update32_body<D, RES, CFL>(a, A, B, zero_dq, LFM_TID(blockIdx.x), LFM_BLK(), cfl_param);
 }
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
@@ -1075,20 +1084,14 @@ __global__ void __launch_bounds__(256) face_flux32_b(const KArgs32 *args, BatchM
     batch_slot(m, r, lb);
     face_flux32_body<D, SOLVER, VISC, false>(args[r], lb * 256 + threadIdx.x);
 }
-template <int D>
+// CFL 1: the last stage, with compute_cfl at each solver's own deltaT
+template <int D, int CFL>
 __global__ void __launch_bounds__(256) cell_update32_b(const KArgs32 *args, BatchMap m, float A, float B,
                                                        int zero_dq) {
     int r, lb;
     batch_slot(m, r, lb);
-    cell_update32_body<D, false>(args[r], A, B, zero_dq, lb * 256 + threadIdx.x, lb);
-}
-// compute_cfl with each solver's own deltaT
-template <int D>
-__global__ void __launch_bounds__(256) cfl_cells32_b(const KArgs32 *args, BatchMap m) {
-    int r, lb;
-    batch_slot(m, r, lb);
     const KArgs32 &a = args[r];
-    cfl_dt_cells32_body<D, 1>(a, *a.dt, lb * 256 + threadIdx.x, lb);
+    cell_update32_body<D, false, CFL>(a, A, B, zero_dq, lb * 256 + threadIdx.x, lb, CFL >= 0 ? *a.dt : 0.0);
 }
 // one block per solver: max of its partials into its CFL scalar
 static __global__ void __launch_bounds__(256) cfl_final32_b(const KArgs32 *args) {
@@ -1118,7 +1121,8 @@ void launch_gradvis32(Solver *s, bool full);
 template <int D>
 void launch_flux32(Solver *s);
 template <int D>
-void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool accumulate_res);
+void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool accumulate_res,
+                     const Solver::CflJob *cfl);
 template <int D>
 void launch_cfl_dt32(Solver *s, int mode, double param, double *out, double *dt_dst);
 

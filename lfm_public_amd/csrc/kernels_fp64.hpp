@@ -490,9 +490,12 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
 // finite value, i.e. a zero the reference adds; it is not formed.
 // STORED: tauMC / sigmaU read from the arrays grad_vis filled (partitions,
 // whose halo payload carries them); otherwise formed here from dudx.
+// In 2D, 3 waves per SIMD: the viscous M2AUSM face then fits 168 VGPRs
+// without spills instead of 170 (2 waves), 82 -> 73 us per launch on 1 M
+// cells; in 3D the same bound spills and is slower.
 // ---------------------------------------------------------------------------
 template <int D, int SOLVER, bool VISC, bool STORED>
-__global__ void __launch_bounds__(256) face_flux(KArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 2 ? 3 : 2))) face_flux(KArgs a) {
     const int f = LFM_TID(blockIdx.x);
     if (f >= a.nf) return;
     const int P = a.fP[f], N = a.fN[f];
@@ -807,14 +810,99 @@ __global__ void __launch_bounds__(256) vis_tensor_field(KArgs a, double *tmc_out
 }
 
 // ---------------------------------------------------------------------------
+// compute_dt / compute_cfl (cfd_v0.cpp:2887-2942) of one cell from its state
+// q: the sum over its faces in slot order, then dt_c (MODE 0) or cfl_c
+// (MODE 1).  Returns the block's starting value combined with it, so a NaN
+// never wins (as `dt_min > dt`).
+// ---------------------------------------------------------------------------
+template <int D, int MODE>
+__device__ __forceinline__ double cfl_cell(const KArgs &a, const int c, const double (&q)[D + 2], double param) {
+    double sum = 0.0;
+    auto add = [&](const double (&S)[D]) {
+        if (MODE == 0) {
+            double sM = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; k++) sM += S[k] * S[k];
+            sM = sqrt(sM);
+            double fl = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; k++) fl += q[k + 1] * S[k] / sM;
+            sum += fabs(fl);
+        } else {
+            double fl = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; k++) fl += q[k + 1] * S[k];
+            sum += fabs(fl);
+        }
+    };
+    // slot-ordered sum; the first slots' area vectors fetched together
+    // (slot lists carry no ghost flag)
+    const int e0 = a.s_off[c], ne = a.s_off[c + 1] - e0;
+    constexpr int U = D == 2 ? 4 : 6;
+    double Sj[U][D];
+#pragma unroll
+    for (int j = 0; j < U; j++) {
+        const int ent = a.s_ent[e0 + (j < ne ? j : ne - 1)];
+        const int f = ent >= 0 ? ent : ~ent;
+#pragma unroll
+        for (int k = 0; k < D; k++) Sj[j][k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
+    }
+#pragma unroll
+    for (int j = 0; j < U; j++)
+        if (j < ne) add(Sj[j]);
+    for (int e = e0 + U; e < e0 + ne; e++) {
+        const int ent = a.s_ent[e];
+        const int f = ent >= 0 ? ent : ~ent;
+        double S[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) S[k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
+        add(S);
+    }
+    double v = MODE == 0 ? 100.0 : 0.0;
+    if (MODE == 0) {
+        const double vol = 1.0 / a.vinv[c];
+        const double dtc = param * q[0] * 2.0 * vol / sum;
+        if (v > dtc) v = dtc;
+    } else {
+        const double cfl = sum * ((1.0 / q[0]) * a.vinv[c] * param);
+        if (v < cfl) v = cfl;
+    }
+    return v;
+}
+
+// Block-wide min (MODE 0) / max (MODE 1) of v into *dst; every thread of the
+// block calls it.  The values are never NaN, so min / max are exact in any
+// order.
+template <int MODE>
+__device__ __forceinline__ void block_extremum(double v, double *dst) {
+    __shared__ double ext[256 / 64];
+    auto pick = [](double x, double o) {
+        if (MODE == 0) return x > o ? o : x;
+        return x < o ? o : x;
+    };
+    for (int off = 32; off > 0; off >>= 1) v = pick(v, __shfl_down(v, off, 64));
+    if ((threadIdx.x & 63) == 0) ext[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = ext[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) r = pick(r, ext[i]);
+        *dst = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Residual gather, sponge, RK update (cfd_v0.cpp:2792-2822).  dq starts from
 // A_rk * dq (prepare_for_RKstep); zero_dq replaces it by 0 for the first
-// stage of a step (prepare_for_timestep zeroed it).
+// stage of a step (prepare_for_timestep zeroed it).  CFL >= 0 (the last
+// stage of a step): the step's compute_dt (0) / compute_cfl (1) of the
+// updated cells, per-block extrema into a.partial.
 // ---------------------------------------------------------------------------
-template <int D, bool RES>
-__global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, int zero_dq) {
+template <int D, bool RES, int CFL>
+__global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, int zero_dq, double cfl_param) {
+    static_assert(!(RES && CFL >= 0), "the residual and the CFL share a.partial");
     const int c = LFM_TID(blockIdx.x);
     __shared__ double red[256 / 64][D + 2];
+    double cv = CFL == 0 ? 100.0 : 0.0;
     double rsq[D + 2];
 #pragma unroll
     for (int k = 0; k < D + 2; k++) rsq[k] = 0.0;
@@ -886,6 +974,7 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
             if (!ghost_state<D>(a, a.ghost_role[g], q, qg)) continue;
             store_q<D>(a.q, a.ntot, a.nc + g, qg);
         }
+        if constexpr (CFL >= 0) cv = cfl_cell<D, CFL>(a, c, q, cfl_param);
         if (RES) {
 #pragma unroll
             for (int k = 0; k < D + 2; k++) {
@@ -913,6 +1002,7 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
             }
         }
     }
+    if constexpr (CFL >= 0) block_extremum<CFL>(cv, a.partial + LFM_BLK());
 }
 
 // Sum of per-block partials in fixed order, one block.
@@ -933,81 +1023,20 @@ static __global__ void __launch_bounds__(256) reduce_sum(const double *partial, 
 }
 
 // ---------------------------------------------------------------------------
-// compute_dt / compute_cfl (cfd_v0.cpp:2887-2942): per cell over its faces in
-// slot order, then min / max.  MODE 0: dt (min, start 100), 1: cfl (max, start 0).
+// compute_dt / compute_cfl (cfd_v0.cpp:2887-2942) as their own launch: per
+// cell over its faces in slot order, then min / max.  MODE 0: dt (min, start
+// 100), 1: cfl (max, start 0).
 // ---------------------------------------------------------------------------
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
     const int c = LFM_TID(blockIdx.x);
-    __shared__ double sh[256];
     double v = MODE == 0 ? 100.0 : 0.0;
     if (c < a.nc) {
         double q[D + 2];
         load_q<D>(a, c, q);
-        double sum = 0.0;
-        auto add = [&](const double (&S)[D]) {
-            if (MODE == 0) {
-                double sM = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; k++) sM += S[k] * S[k];
-                sM = sqrt(sM);
-                double fl = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; k++) fl += q[k + 1] * S[k] / sM;
-                sum += fabs(fl);
-            } else {
-                double fl = 0.0;
-#pragma unroll
-                for (int k = 0; k < D; k++) fl += q[k + 1] * S[k];
-                sum += fabs(fl);
-            }
-        };
-        // slot-ordered sum; the first slots' area vectors fetched together
-        // (slot lists carry no ghost flag)
-        const int e0 = a.s_off[c], ne = a.s_off[c + 1] - e0;
-        constexpr int U = D == 2 ? 4 : 6;
-        double Sj[U][D];
-#pragma unroll
-        for (int j = 0; j < U; j++) {
-            const int ent = a.s_ent[e0 + (j < ne ? j : ne - 1)];
-            const int f = ent >= 0 ? ent : ~ent;
-#pragma unroll
-            for (int k = 0; k < D; k++) Sj[j][k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
-        }
-#pragma unroll
-        for (int j = 0; j < U; j++)
-            if (j < ne) add(Sj[j]);
-        for (int e = e0 + U; e < e0 + ne; e++) {
-            const int ent = a.s_ent[e];
-            const int f = ent >= 0 ? ent : ~ent;
-            double S[D];
-#pragma unroll
-            for (int k = 0; k < D; k++) S[k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
-            add(S);
-        }
-        if (MODE == 0) {
-            const double vol = 1.0 / a.vinv[c];
-            const double dtc = param * q[0] * 2.0 * vol / sum;
-            if (v > dtc) v = dtc;  // NaN never wins (as `dt_min > dt`)
-        } else {
-            const double cfl = sum * ((1.0 / q[0]) * a.vinv[c] * param);
-            if (v < cfl) v = cfl;
-        }
+        v = cfl_cell<D, MODE>(a, c, q, param);
     }
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-            const double o = sh[threadIdx.x + st];
-            if (MODE == 0) {
-                if (sh[threadIdx.x] > o) sh[threadIdx.x] = o;
-            } else {
-                if (sh[threadIdx.x] < o) sh[threadIdx.x] = o;
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a.partial[LFM_BLK()] = sh[0];
+    block_extremum<MODE>(v, a.partial + LFM_BLK());
 }
 
 template <int MODE>

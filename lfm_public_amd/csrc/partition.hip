@@ -330,7 +330,7 @@ int lfm_part_group_run(lfm_part_group *g, int nsteps) {
             for (Solver *p : g->parts) p->stage_a(rk);
             rc = exchange(g, LFM_HALO_VISCOUS);
             if (rc) break;
-            for (Solver *p : g->parts) p->stage_b(rk);
+            for (Solver *p : g->parts) p->stage_b(rk, true, !g->adjust);
             rc = exchange(g, LFM_HALO_SOLVARS);
         }
         if (rc) break;

@@ -209,16 +209,27 @@ void launch_flux32(Solver *s) {
 }
 
 template <int D>
-void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool accumulate_res) {
+void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool accumulate_res,
+                     const Solver::CflJob *cfl) {
     const int nb = nblocks(s->L.nc);
     if (res) {
-        hipLaunchKernelGGL((cell_update32<D, true>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
-                           (float)B, (int)zero_dq);
+        hipLaunchKernelGGL((cell_update32<D, true, -1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
+                           (float)B, (int)zero_dq, 0.0);
         hipLaunchKernelGGL(reduce_sum, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, D + 2, s->d_res,
                            (int)accumulate_res);
+    } else if (cfl && cfl->mode == 0) {
+        hipLaunchKernelGGL((cell_update32<D, false, 0>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
+                           (float)B, (int)zero_dq, cfl->param);
+        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, cfl->out,
+                           cfl->dt_dst);
+    } else if (cfl) {
+        hipLaunchKernelGGL((cell_update32<D, false, 1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
+                           (float)B, (int)zero_dq, cfl->param);
+        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, cfl->out,
+                           cfl->dt_dst);
     } else {
-        hipLaunchKernelGGL((cell_update32<D, false>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
-                           (float)B, (int)zero_dq);
+        hipLaunchKernelGGL((cell_update32<D, false, -1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
+                           (float)B, (int)zero_dq, 0.0);
     }
 }
 
@@ -350,8 +361,8 @@ template void launch_gradvis32<2>(Solver *, bool);
 template void launch_gradvis32<3>(Solver *, bool);
 template void launch_flux32<2>(Solver *);
 template void launch_flux32<3>(Solver *);
-template void launch_update32<2>(Solver *, double, double, bool, bool, bool);
-template void launch_update32<3>(Solver *, double, double, bool, bool, bool);
+template void launch_update32<2>(Solver *, double, double, bool, bool, bool, const Solver::CflJob *);
+template void launch_update32<3>(Solver *, double, double, bool, bool, bool, const Solver::CflJob *);
 template void launch_cfl_dt32<2>(Solver *, int, double, double *, double *);
 template void launch_cfl_dt32<3>(Solver *, int, double, double *, double *);
 

@@ -103,18 +103,33 @@ struct Solver {
 
     ~Solver();
 
+    // the end-of-step reduction: compute_dt (mode 0, min into out and, when
+    // dt_dst is set, the deltaT the next step uses) or compute_cfl (mode 1)
+    struct CflJob {
+        int mode;
+        double param;
+        double *out, *dt_dst;
+    };
+    CflJob cfl_job(bool own_dt);
+    bool cfl_fused = false;  // the last update of this step formed it
+
     // stage pieces on the handle's stream
     void bc();
     void gradvis(bool full);
     void flux();
-    void update(double A, double B, bool zero_dq, bool res, bool accumulate_res);
+    // cfl: the update also forms that reduction from the updated cells (it
+    // reads only a cell's own state and its faces' areas); not with res
+    void update(double A, double B, bool zero_dq, bool res, bool accumulate_res, const CflJob *cfl = nullptr);
     void cfl_dt(int mode, double param, double *out, double *dt_dst);
     void step();
     // pieces of step(): stage part A (ghost fill if stale, gradients and
     // viscous tensors), part B (fluxes, update), and the end-of-step CFL/dt
-    // reduction (dt written to the own deltaT when own_dt)
+    // reduction (dt written to the own deltaT when own_dt).  stage_b of the
+    // last stage forms that reduction in the update when fuse_cfl is set,
+    // the stage computes no residual and the path is FP32 or 2D; end_step
+    // then launches nothing.
     void stage_a(int rk);
-    void stage_b(int rk);
+    void stage_b(int rk, bool fuse_cfl = false, bool own_dt = true);
     void end_step(bool own_dt);
     void zero_state_for_step();
 
