@@ -166,12 +166,15 @@ int lfm_compute_cfl(lfm_solver *s, double dt, double *cfl_out);
 int lfm_run(lfm_solver *s, int nsteps);
 /* Wait for all queued work of the handle. */
 int lfm_synchronize(lfm_solver *s);
-/* Time n steps with HIP events on the handle's stream (ms). */
+/* Time n steps (as lfm_run launches them, without post-processing) with
+ * HIP events on the handle's stream (ms). */
 int lfm_time_steps(lfm_solver *s, int nsteps, double *ms_out);
 
-/* Time n steps with HIP events around every kernel group on the handle's
- * stream: ms_per_group[5] = bc fill, gradients+viscous, face fluxes,
- * cell update (+ residual reduction), CFL/dt reduction; ms_total = whole. */
+/* Time n steps launched kernel by kernel with HIP events around every
+ * kernel group on the handle's stream: ms_per_group[5] = bc fill,
+ * gradients+viscous, face fluxes, cell update (+ residual reduction; the
+ * last stage's also forms the CFL/dt extrema and their final reduction
+ * where fused), CFL/dt reduction; ms_total = whole. */
 int lfm_profile_steps(lfm_solver *s, int nsteps, double *ms_per_group, double *ms_total);
 
 /* Fields in original cell order, SoA [comp][cell], FP64 on the host. */
