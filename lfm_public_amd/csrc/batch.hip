@@ -15,13 +15,6 @@
 
 namespace lfm {
 
-#define HIP_TRY(expr)                                                                            \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail(LFM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
-    } while (0)
-
 struct Batch {
     hipStream_t stream = nullptr;
     std::vector<Solver *> members;

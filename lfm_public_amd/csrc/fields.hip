@@ -9,13 +9,6 @@
 
 namespace lfm {
 
-#define HIP_TRY(expr)                                                                            \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail(LFM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
-    } while (0)
-
 // ISolver::prepare_for_timestep: dq = 0, RES = 0
 static __global__ void __launch_bounds__(256) fill_zero32(float *p, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

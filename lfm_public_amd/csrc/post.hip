@@ -34,13 +34,6 @@
 
 namespace lfm {
 
-#define HIP_TRY(expr)                                                                            \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail(LFM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
-    } while (0)
-
 static constexpr int kPostBlock = 256;
 static constexpr int kHistRows = 1024;  // force rows kept on the device before a drain
 

@@ -10,13 +10,6 @@
 
 namespace lfm {
 
-#define HIP_TRY(expr)                                                                            \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail(LFM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
-    } while (0)
-
 static constexpr int kBlock = 256;
 static inline int nblocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 

@@ -23,6 +23,16 @@ void free_post(PostState *p);
 
 int fail(int code, const std::string &msg);
 
+// Returns the LFM_EDEVICE status (with the failing call and HIP's message
+// recorded as the last error) from the enclosing function when a HIP call
+// fails.
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(LFM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
 // Serialises device set-up (allocation, synchronous uploads) of new solvers
 // with hipGraph capture on other threads: a synchronous copy issued while a
 // capture is open would invalidate it.
