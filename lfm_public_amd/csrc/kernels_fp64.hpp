@@ -1039,36 +1039,39 @@ __global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
     block_extremum<MODE>(v, a.partial + LFM_BLK());
 }
 
+// Final min / max over the per-block values: one block of kFinalBlock
+// threads, each with 16 loads in flight per round, so the latency of a few
+// rounds (65 536 partials on 16.8 M cells: 4) bounds it rather than a long
+// dependent chain.  Min / max are exact in any order.
+constexpr int kFinalBlock = 1024;
 template <int MODE>
-static __global__ void __launch_bounds__(256) cfl_dt_final(const double *partial, int nblk, double *out, double *dt_dst) {
-    __shared__ double sh[256];
+static __global__ void __launch_bounds__(kFinalBlock) cfl_dt_final(const double *partial, int nblk, double *out,
+                                                                  double *dt_dst) {
+    auto pick = [](double x, double o) {
+        if (MODE == 0) return x > o ? o : x;
+        return x < o ? o : x;
+    };
+    constexpr int kLoads = 16;
     double v = MODE == 0 ? 100.0 : 0.0;
-    // min / max are exact in any order: keep several loads in flight
-#pragma unroll 8
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
-        const double o = partial[i];
-        if (MODE == 0) {
-            if (v > o) v = o;
-        } else {
-            if (v < o) v = o;
+    for (int i0 = 0; i0 < nblk; i0 += kLoads * kFinalBlock) {
+        double o[kLoads];
+#pragma unroll
+        for (int j = 0; j < kLoads; j++) {
+            const int i = i0 + j * kFinalBlock + (int)threadIdx.x;
+            o[j] = i < nblk ? partial[i] : v;
         }
+#pragma unroll
+        for (int j = 0; j < kLoads; j++) v = pick(v, o[j]);
     }
-    sh[threadIdx.x] = v;
+    for (int off = 32; off > 0; off >>= 1) v = pick(v, __shfl_down(v, off, 64));
+    __shared__ double w[kFinalBlock / 64];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) {
-            const double o = sh[threadIdx.x + st];
-            if (MODE == 0) {
-                if (sh[threadIdx.x] > o) sh[threadIdx.x] = o;
-            } else {
-                if (sh[threadIdx.x] < o) sh[threadIdx.x] = o;
-            }
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        *out = sh[0];
-        if (dt_dst) *dt_dst = sh[0];
+        double r = w[0];
+        for (int i = 1; i < kFinalBlock / 64; i++) r = pick(r, w[i]);
+        *out = r;
+        if (dt_dst) *dt_dst = r;
     }
 }
 

@@ -213,12 +213,12 @@ static void launch_update64(Solver *s, double A, double B, bool zero_dq, bool re
     } else if (cfl && cfl->mode == 0) {
         hipLaunchKernelGGL((cell_update<D, false, 0>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, A, B,
                            (int)zero_dq, cfl->param);
-        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kBlock), 0, s->stream, s->k64.partial, nb, cfl->out,
+        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k64.partial, nb, cfl->out,
                            cfl->dt_dst);
     } else if (cfl) {
         hipLaunchKernelGGL((cell_update<D, false, 1>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, A, B,
                            (int)zero_dq, cfl->param);
-        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kBlock), 0, s->stream, s->k64.partial, nb, cfl->out,
+        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k64.partial, nb, cfl->out,
                            cfl->dt_dst);
     } else {
         hipLaunchKernelGGL((cell_update<D, false, -1>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, A, B,
@@ -231,10 +231,10 @@ static void launch_cfl_dt64(Solver *s, int mode, double param, double *out, doub
     const int nb = nblocks(s->L.nc);
     if (mode == 0) {
         hipLaunchKernelGGL((cfl_dt_cells<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, param);
-        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kBlock), 0, s->stream, s->k64.partial, nb, out, dt_dst);
+        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k64.partial, nb, out, dt_dst);
     } else {
         hipLaunchKernelGGL((cfl_dt_cells<D, 1>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, param);
-        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kBlock), 0, s->stream, s->k64.partial, nb, out, dt_dst);
+        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k64.partial, nb, out, dt_dst);
     }
 }
 

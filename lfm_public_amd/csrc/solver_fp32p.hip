@@ -213,12 +213,12 @@ void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool
     } else if (cfl && cfl->mode == 0) {
         hipLaunchKernelGGL((cell_update32<D, false, 0>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
                            (float)B, (int)zero_dq, cfl->param);
-        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, cfl->out,
+        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k32->partial, nb, cfl->out,
                            cfl->dt_dst);
     } else if (cfl) {
         hipLaunchKernelGGL((cell_update32<D, false, 1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
                            (float)B, (int)zero_dq, cfl->param);
-        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, cfl->out,
+        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k32->partial, nb, cfl->out,
                            cfl->dt_dst);
     } else {
         hipLaunchKernelGGL((cell_update32<D, false, -1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
@@ -231,10 +231,10 @@ void launch_cfl_dt32(Solver *s, int mode, double param, double *out, double *dt_
     const int nb = nblocks(s->L.nc);
     if (mode == 0) {
         hipLaunchKernelGGL((cfl_dt_cells32<D, 0>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, param);
-        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, out, dt_dst);
+        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k32->partial, nb, out, dt_dst);
     } else {
         hipLaunchKernelGGL((cfl_dt_cells32<D, 1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, param);
-        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kBlock), 0, s->stream, s->k32->partial, nb, out, dt_dst);
+        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k32->partial, nb, out, dt_dst);
     }
 }
 
