@@ -57,14 +57,6 @@ struct KArgs32 {
 
 #define LERP32(w, a, b) ((w) * (a) + (1.0f - (w)) * (b))
 
-// Faces per cell whose gathers are issued together before any arithmetic
-// (the cell's faces beyond it are gathered one at a time): every cell of a
-// 2D quad/triangle mesh and of a 3D hex mesh fits.
-template <int D>
-struct GatherUnroll {
-    static constexpr int value = D == 2 ? 4 : 6;
-};
-
 // FP32 reciprocal, quotient and square root on the hardware instructions
 // (v_rcp_f32, v_sqrt_f32: 1 ulp).  Spelled as target builtins so that every
 // kernel instantiating a body, single or batched, selects the same
@@ -340,7 +332,7 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
         }
     };
     const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
-    // the first kGatherUnroll faces: every load issued before the sums (a
+    // the first GatherUnroll faces: every load issued before the sums (a
     // short list repeats its last face, whose terms are then not added)
     constexpr int U = GatherUnroll<D>::value;
     bool isP[U];

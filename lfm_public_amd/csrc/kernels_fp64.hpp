@@ -80,6 +80,14 @@ __device__ __forceinline__ double dotD(const double *x, const double *y) {
     return s;
 }
 
+// Faces per cell whose gathers are issued together before any arithmetic
+// (the cell's faces beyond it are gathered one at a time): every cell of a
+// 2D quad/triangle mesh and of a 3D hex mesh fits.
+template <int D>
+struct GatherUnroll {
+    static constexpr int value = D == 2 ? 4 : 6;
+};
+
 // State layout: components 0-3 of a cell (or ghost) i as one 32-byte
 // record, q[4 i + k], then component 4 (the 3D energy) as q[4 ntot + i]: a
 // gathered neighbour state is two 16-byte loads (three in 3D).
@@ -405,7 +413,7 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
     const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
     // the first faces' gathers issued together before the sums (a short list
     // repeats its last face, whose terms are then not added)
-    constexpr int U = D == 2 ? 4 : 6;
+    constexpr int U = GatherUnroll<D>::value;
     bool isPj[U];
     double wj[U], Sj[U][D], qj[U][D + 2];
 #pragma unroll
@@ -838,7 +846,7 @@ __device__ __forceinline__ double cfl_cell(const KArgs &a, const int c, const do
     // slot-ordered sum; the first slots' area vectors fetched together
     // (slot lists carry no ghost flag)
     const int e0 = a.s_off[c], ne = a.s_off[c + 1] - e0;
-    constexpr int U = D == 2 ? 4 : 6;
+    constexpr int U = GatherUnroll<D>::value;
     double Sj[U][D];
 #pragma unroll
     for (int j = 0; j < U; j++) {
@@ -935,7 +943,7 @@ __global__ void __launch_bounds__(256) cell_update(KArgs a, double A, double B, 
             }
         };
         // the first faces' fluxes fetched together, then added in gather order
-        constexpr int U = D == 2 ? 4 : 6;
+        constexpr int U = GatherUnroll<D>::value;
         int ent[U];
         double fr[U][D + 2];
 #pragma unroll
