@@ -114,3 +114,64 @@ def test_generated_meshes_follow_openfoam_conventions(make):
     np.add.at(tot, m.owner, fa)
     np.add.at(tot, m.neighbour, -fa[: m.n_internal])
     assert np.abs(tot).max() < 1e-12 * np.abs(fa).max() * 10
+
+
+def _mutated(mesh, **kw):
+    import dataclasses
+    return dataclasses.replace(mesh, **kw)
+
+
+def _heptagon_cell():
+    """One 2D cell bounded by seven side faces (a wall) and its two empty
+    faces: seven valid faces, above the reference's FACE_CNT maximum of 6
+    (src/mesh_solver.cpp:56-74)."""
+    ang = 2 * np.pi * np.arange(7) / 7
+    ring = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    pts = np.concatenate([np.c_[ring, np.zeros(7)], np.c_[ring, np.ones(7)]])
+    faces = [np.array([k, (k + 1) % 7, 7 + (k + 1) % 7, 7 + k]) for k in range(7)]
+    faces += [np.arange(7)[::-1], 7 + np.arange(7)]
+    face_off = np.cumsum([0] + [len(f) for f in faces]).astype(np.int32)
+    return meshgen.PolyMesh(points=pts, face_off=face_off, face_pts=np.concatenate(faces).astype(np.int32),
+                            owner=np.zeros(9, np.int32), neighbour=np.zeros(0, np.int32), n_cells=1,
+                            patches=[meshgen.Patch("walls", "wall", 0, 7),
+                                     meshgen.Patch("frontAndBack", "empty", 7, 2)], dim=2)
+
+
+def test_malformed_meshes_rejected_before_the_device():
+    """The builder's input checks (mesh_builder.cpp), each with its message:
+    the reference trusts OpenFOAM's mesh and would index out of bounds."""
+    case = sc.vortex(6)
+    m = case.mesh
+    nc, ni = m.n_cells, m.n_internal
+
+    def rejected(mesh, msg, p=None, T=None, U=None):
+        n = mesh.n_cells
+        with pytest.raises(capi.LfmError) as e:
+            capi.GpuSolver(mesh, case.params, case.p[:n] if p is None else p, case.T[:n] if T is None else T,
+                           case.U[:n] if U is None else U)
+        assert e.value.code == capi.LFM_EINVAL
+        assert msg in str(e.value), (msg, str(e.value))
+
+    own = m.owner.copy()
+    own[ni] = nc
+    rejected(_mutated(m, owner=own), "owner out of range")
+    nb = m.neighbour.copy()
+    nb[0] = -1
+    rejected(_mutated(m, neighbour=nb), "neighbour out of range")
+    off = m.face_off.copy()
+    off[1] = off[0] + 2  # face 0 keeps two points, face 1 takes the rest
+    rejected(_mutated(m, face_off=off), "fewer than 3 points")
+    last = m.patches[-1]
+    longer = m.patches[:-1] + [meshgen.Patch(last.name, last.type, last.start, last.size + 1)]
+    rejected(_mutated(m, patches=longer), "out of bounds")
+    shorter = m.patches[:-1] + [meshgen.Patch(last.name, last.type, last.start, last.size - 1)]
+    rejected(_mutated(m, patches=shorter), "outside every patch")
+    with pytest.raises(capi.LfmError) as e:
+        capi.GpuSolver.from_case(case, roles=[99] * len(m.patches))
+    assert "unknown patch role" in str(e.value)
+    h = _heptagon_cell()
+    rejected(h, "more than 6 valid faces", p=np.ones(1), T=np.ones(1), U=np.zeros((1, 3)))
+    empty = meshgen.PolyMesh(points=m.points, face_off=np.zeros(1, np.int32), face_pts=np.zeros(0, np.int32),
+                             owner=np.zeros(0, np.int32), neighbour=np.zeros(0, np.int32), n_cells=0, patches=[],
+                             dim=2)
+    rejected(empty, "empty mesh", p=np.zeros(0), T=np.zeros(0), U=np.zeros((0, 3)))
