@@ -145,13 +145,25 @@ struct Lexer {
         i += (size_t)(e - b);
         return v;
     }
+    // a list size: non-negative and within the 32-bit labels of this
+    // library; with item_bytes > 0 (a list written out item by item), also
+    // at most what the rest of the file can hold
+    long count(size_t item_bytes) {
+        const long n = lab();
+        if (n < 0) perr(file, "negative list size");
+        if (n > INT32_MAX) perr(file, "list size beyond the 32-bit label range");
+        const size_t left = i < s.size() ? s.size() - i : 0;
+        if (item_bytes > 0 && (size_t)n > left / item_bytes)
+            perr(file, "list of " + std::to_string(n) + " items is longer than the file");
+        return n;
+    }
     // binary list payload: the opening bracket (after white space only --
     // the bytes may contain anything), `bytes` raw bytes, the closing one
     const char *raw(size_t bytes, char open, char close) {
         while (i < s.size() && std::isspace((unsigned char)s[i])) i++;
         if (i >= s.size() || s[i] != open) perr(file, std::string("expected '") + open + "' before binary data");
         const char *p = s.data() + i + 1;
-        if (i + 1 + bytes >= s.size() || s[i + 1 + bytes] != close)
+        if (bytes >= s.size() - (i + 1) || s[i + 1 + bytes] != close)
             perr(file, "binary list of " + std::to_string(bytes) + " bytes is truncated");
         i += bytes + 2;
         return p;
@@ -405,11 +417,14 @@ int32_t to_label32(Lexer &L, long x) {
 
 // "N ( a b ... )" or "N{v}" of labels (raw bytes in a binary file)
 std::vector<int32_t> read_labels(Lexer &L) {
-    long n = L.lab();
+    long n = L.count(0);
     std::vector<int32_t> v;
-    if (n < 0) perr(L.file, "negative list size");
+    const bool uniform = L.bracket() == '{';
+    // item by item: at least a label's bytes (binary) or one character each
+    if (!uniform && (size_t)n > (L.s.size() - std::min(L.i, L.s.size())) / (L.binary ? L.label_bytes : 1))
+        perr(L.file, "list of " + std::to_string(n) + " labels is longer than the file");
     if (L.binary) {
-        const bool uni = L.bracket() == '{';
+        const bool uni = uniform;
         const char *p = uni ? L.raw((size_t)L.label_bytes, '{', '}') : L.raw((size_t)n * L.label_bytes, '(', ')');
         v.resize((size_t)n);
         for (long k = 0; k < n; k++) v[k] = to_label32(L, L.label_at(p, uni ? 0 : (size_t)k));
@@ -642,7 +657,7 @@ static void read_mesh(lfm_case &c) {
         ListFile lf;
         open_list(pm + "points", lf);
         Lexer &L = *lf.L;
-        long n = L.lab();
+        long n = L.count(L.binary ? 3 * (size_t)L.scalar_bytes : 7);  // "(x y z)"
         c.points.resize((size_t)n * 3);
         if (L.binary) {
             const char *p = L.raw((size_t)n * 3 * L.scalar_bytes, '(', ')');
@@ -665,15 +680,19 @@ static void read_mesh(lfm_case &c) {
         if (header_word(lf.header, "class") == "faceCompactList" || L.binary) {
             std::vector<int32_t> off = read_labels(L);
             std::vector<int32_t> pts = read_labels(L);
+            if (off.empty() || off[0] != 0 || (size_t)off.back() != pts.size())
+                perr(pm + "faces", "face offsets do not span the point labels");
+            for (size_t k = 1; k < off.size(); k++)
+                if (off[k] < off[k - 1]) perr(pm + "faces", "face offsets decrease");
             c.face_off = off;
             c.face_pts = pts;
         } else {
-            long n = L.lab();
+            long n = L.count(5);  // "k(...)"
             L.expect('(');
             c.face_off.assign(1, 0);
             c.face_off.reserve((size_t)n + 1);
             for (long i = 0; i < n; i++) {
-                long k = L.lab();
+                long k = L.count(2);
                 L.expect('(');
                 for (long j = 0; j < k; j++) c.face_pts.push_back((int32_t)L.lab());
                 L.expect(')');
@@ -691,13 +710,15 @@ static void read_mesh(lfm_case &c) {
         int32_t m = -1;
         for (int32_t o : c.owner) m = std::max(m, o);
         for (int32_t o : c.neighbour) m = std::max(m, o);
+        // every cell owns or neighbours at least one face
+        if ((long)m >= nf) perr(pm + "owner", "cell label " + std::to_string(m) + " beyond the face count");
         c.n_cells = m + 1;
     }
     {
         ListFile lf;
         open_list(pm + "boundary", lf);
         Lexer &L = *lf.L;
-        long n = L.lab();
+        long n = L.count(3);  // "p{}"
         L.expect('(');
         for (long i = 0; i < n; i++) {
             std::string name = L.next();
@@ -877,6 +898,8 @@ int lfm_case_open(const char *case_dir, int32_t rank, lfm_case **out) {
         return fail(LFM_EINVAL, e.msg);
     } catch (const std::bad_alloc &) {
         return fail(LFM_ENOMEM, "out of host memory reading the case");
+    } catch (const std::exception &e) {
+        return fail(LFM_EINVAL, c->dir + ": " + e.what());
     }
     *out = c.release();
     return LFM_OK;
@@ -946,6 +969,8 @@ int lfm_case_dict_scalar(lfm_case *c, const char *dict, const char *subdict, con
         *out = dict_scalar(f, subdict ? subdict : "", key, dflt, allow_default != 0);
     } catch (const ParseError &e) {
         return fail(LFM_EINVAL, e.msg);
+    } catch (const std::exception &e) {
+        return fail(LFM_EINVAL, e.what());
     }
     return LFM_OK;
 }
@@ -965,6 +990,8 @@ int lfm_case_post(lfm_case *c, lfm_post *post) {
         post->average_start = (int32_t)((t_avg - c->start) / c->dt);
     } catch (const ParseError &e) {
         return fail(LFM_EINVAL, e.msg);
+    } catch (const std::exception &e) {
+        return fail(LFM_EINVAL, e.what());
     }
     post->time = c->start;
     return LFM_OK;
@@ -980,6 +1007,8 @@ int lfm_case_write_scalar(lfm_case *c, double time, const char *name, const char
                     calculated_boundary(*c));
     } catch (const ParseError &e) {
         return fail(LFM_EINVAL, e.msg);
+    } catch (const std::exception &e) {
+        return fail(LFM_EINVAL, e.what());
     }
     return LFM_OK;
 }
@@ -1015,6 +1044,8 @@ int lfm_case_write_time(lfm_case *c, double time, const double *q, int32_t dim) 
         write_field(d + "/E", "volScalarField", "E", "[0 2 -2 0 0 0 0]", E, 1, bnd);
     } catch (const ParseError &e) {
         return fail(LFM_EINVAL, e.msg);
+    } catch (const std::exception &e) {
+        return fail(LFM_EINVAL, e.what());
     }
     return LFM_OK;
 }

@@ -178,7 +178,7 @@ std::string build_layout(const lfm_polymesh &m, const lfm_partition *part, const
     std::vector<int32_t> fpatch(nf, -1);
     for (int pch = 0; pch < m.n_patches; pch++) {
         const int s = m.patch_start[pch], n = m.patch_size[pch];
-        if (s < m.n_internal || n < 0 || s + n > nf) return "patch range out of bounds";
+        if (s < m.n_internal || n < 0 || (long)s + n > nf) return "patch range out of bounds";
         for (int f = s; f < s + n; f++) fpatch[f] = pch;
         int role = m.patch_role[pch];
         if (role < LFM_BC_UNSET || role > LFM_BC_PROCESSOR) return "unknown patch role";
@@ -195,10 +195,14 @@ std::string build_layout(const lfm_polymesh &m, const lfm_partition *part, const
     }
     for (int f = m.n_internal; f < nf; f++)
         if (fpatch[f] < 0) return "boundary face outside every patch";
+    if (m.n_patches < 0) return "negative patch count";
+    if (m.n_points <= 0 || m.face_off[0] != 0) return "face offsets must start at 0 over a non-empty point list";
     for (int f = 0; f < nf; f++) {
         if (m.owner[f] < 0 || m.owner[f] >= nc) return "owner out of range";
-        if (m.face_off[f + 1] - m.face_off[f] < 3) return "face with fewer than 3 points";
+        if ((long)m.face_off[f + 1] - m.face_off[f] < 3) return "face with fewer than 3 points";
     }
+    for (int32_t k = 0; k < m.face_off[nf]; k++)
+        if (m.face_pts[k] < 0 || m.face_pts[k] >= m.n_points) return "face point label out of range";
     for (int f = 0; f < m.n_internal; f++)
         if (m.neighbour[f] < 0 || m.neighbour[f] >= nc) return "neighbour out of range";
 
@@ -614,3 +618,18 @@ std::string halo_import(SolverLayout &L, int ni, int n, const int32_t *key, cons
 }
 
 }  // namespace lfm
+
+// Patch role from OpenFOAM patch type and name (cfd_v0.cpp:983-1002).
+extern "C" int lfm_patch_role(const char *type, const char *name) {
+    if (!type || !name) return LFM_BC_UNSET;
+    const std::string t(type), n(name);
+    if (t == "empty") return LFM_BC_EMPTY;
+    if (t == "cyclic") return LFM_BC_CYCLIC;
+    if (t == "processor") return LFM_BC_PROCESSOR;
+    if (t == "wall") return LFM_BC_WALL;
+    if (t == "patch") {
+        if (n == "inlet" || n == "Inlet" || n == "Inflow" || n == "inflow") return LFM_BC_INLET;
+        if (n == "outlet" || n == "Outlet" || n == "Outflow" || n == "outflow") return LFM_BC_OUTLET;
+    }
+    return LFM_BC_UNSET;
+}

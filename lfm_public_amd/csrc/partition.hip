@@ -172,7 +172,12 @@ int lfm_halo_import(lfm_solver *h, int32_t rank, int32_t n_faces, const int32_t 
     if (ni < 0) return fail(LFM_EINVAL, "rank is not a neighbour of this partition");
     for (int j = 0; j < n_faces; j++)
         if (cells[j] < 0) return fail(LFM_EINVAL, "negative cell index in the set-up message");
-    std::string err = lfm::halo_import(s->L, ni, n_faces, keys, cells, disp);
+    std::string err;
+    try {
+        err = lfm::halo_import(s->L, ni, n_faces, keys, cells, disp);
+    } catch (const std::exception &e) {
+        return fail(LFM_ENOMEM, std::string("applying the set-up message: ") + e.what());
+    }
     if (!err.empty()) return fail(LFM_EINVAL, err);
     std::lock_guard<std::mutex> lock(lfm::device_setup_mutex());
     int rc = s->halo_upload_geometry();

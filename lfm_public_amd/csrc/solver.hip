@@ -388,20 +388,6 @@ extern "C" {
 
 const char *lfm_last_error(void) { return lfm::g_last_error.c_str(); }
 
-int lfm_patch_role(const char *type, const char *name) {
-    if (!type || !name) return LFM_BC_UNSET;
-    const std::string t(type), n(name);
-    if (t == "empty") return LFM_BC_EMPTY;
-    if (t == "cyclic") return LFM_BC_CYCLIC;
-    if (t == "processor") return LFM_BC_PROCESSOR;
-    if (t == "wall") return LFM_BC_WALL;
-    if (t == "patch") {
-        if (n == "inlet" || n == "Inlet" || n == "Inflow" || n == "inflow") return LFM_BC_INLET;
-        if (n == "outlet" || n == "Outlet" || n == "Outflow" || n == "outflow") return LFM_BC_OUTLET;
-    }
-    return LFM_BC_UNSET;
-}
-
 int lfm_solver_create(const lfm_polymesh *mesh, const lfm_params *prm, const double *p, const double *T,
                       const double *U3, const double *alpha, lfm_solver **out) {
     return lfm_solver_create_part(mesh, nullptr, prm, p, T, U3, alpha, out);
@@ -418,7 +404,14 @@ int lfm_solver_create_part(const lfm_polymesh *mesh, const lfm_partition *part, 
         return fail(LFM_EINVAL, "precision must be 64 or 32");
     auto s = std::make_unique<Solver>();
     s->prm = *prm;
-    std::string err = lfm::build_layout(*mesh, part, *prm, p, T, U3, alpha, s->L);
+    std::string err;
+    try {
+        err = lfm::build_layout(*mesh, part, *prm, p, T, U3, alpha, s->L);
+    } catch (const std::bad_alloc &) {
+        return fail(LFM_ENOMEM, "out of host memory building the solver layout");
+    } catch (const std::exception &e) {
+        return fail(LFM_EINVAL, std::string("building the solver layout: ") + e.what());
+    }
     if (!err.empty()) return fail(LFM_EINVAL, err);
     s->gas = lfm::derive_gas(*prm);
     s->nstages = lfm::rk_tables(prm->rk_order, s->A, s->B);

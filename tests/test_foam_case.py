@@ -286,3 +286,76 @@ def test_write_scalar(tmp_path):
     assert np.array_equal(np.array(body.split(), float), v)
     assert "bottom\n    {\n        type            calculated;" in txt
     assert "frontAndBack" not in txt or "type            empty" in txt
+
+
+def _replace_count(path, new):
+    """Replaces the list size before the first '(' of a polyMesh list file."""
+    import re
+    txt = open(path, "rb").read().decode("latin1")
+    m = re.search(r"\n(\d+)\s*\n\(", txt)
+    assert m, path
+    open(path, "wb").write((txt[:m.start(1)] + new + txt[m.end(1):]).encode("latin1"))
+
+
+def _replace_label(path, k, new):
+    """Replaces the k-th label of an ASCII labelList file."""
+    import re
+    txt = open(path).read()
+    m = re.search(r"\n\d+\s*\n\(", txt)
+    body = txt[m.end():]
+    labs = list(re.finditer(r"-?\d+", body))
+    a, b = labs[k].span()
+    open(path, "w").write(txt[:m.end()] + body[:a] + str(new) + body[b:])
+
+
+@pytest.mark.parametrize("file,change,msg", [
+    ("points", ("count", "-5"), "negative list size"),
+    ("points", ("count", "1000000000"), "longer than the file"),
+    ("faces", ("count", "-1"), "negative list size"),
+    ("owner", ("count", "99999999"), "longer than the file"),
+    ("owner", ("label", 3, 999999), "beyond the face count"),
+])
+def test_corrupted_mesh_files_rejected(tmp_path, file, change, msg):
+    """A corrupted list size or label is an error with a message, not an
+    allocation of its size or an out-of-bounds access."""
+    case = sc.cylinder(24)
+    d = str(tmp_path)
+    foam.write_case(d, case, end_steps=4)
+    path = os.path.join(d, "constant", "polyMesh", file)
+    if change[0] == "count":
+        _replace_count(path, change[1])
+    else:
+        _replace_label(path, change[1], change[2])
+    with pytest.raises(capi.LfmError) as e:
+        foam.FoamCase(d)
+    assert e.value.code == capi.LFM_EINVAL
+    assert msg in str(e.value), str(e.value)
+
+
+def test_truncated_binary_list_rejected(tmp_path):
+    case = sc.cylinder(24)
+    d = str(tmp_path)
+    foam.write_case(d, case, end_steps=4, binary=True)
+    path = os.path.join(d, "constant", "polyMesh", "points")
+    data = open(path, "rb").read()
+    open(path, "wb").write(data[: len(data) // 2])
+    with pytest.raises(capi.LfmError) as e:
+        foam.FoamCase(d)
+    assert "truncated" in str(e.value) or "end of file" in str(e.value)
+
+
+def test_face_point_label_out_of_range_rejected(tmp_path):
+    """Point labels are checked against the point list before any geometry
+    is formed (the layout build precedes every device allocation)."""
+    case = sc.cylinder(24)
+    d = str(tmp_path)
+    foam.write_case(d, case, end_steps=4)
+    path = os.path.join(d, "constant", "polyMesh", "faces")
+    txt = open(path).read()
+    i = txt.index("4(") + 2
+    open(path, "w").write(txt[:i] + "99999" + txt[txt.index(" ", i):])
+    fc = foam.FoamCase(d)
+    with pytest.raises(capi.LfmError) as e:
+        fc.solver()
+    assert "face point label out of range" in str(e.value)
+    fc.close()
