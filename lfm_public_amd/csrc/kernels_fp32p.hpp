@@ -57,14 +57,16 @@ struct KArgs32 {
 
 #define LERP32(w, a, b) ((w) * (a) + (1.0f - (w)) * (b))
 
-// FP32 reciprocal, quotient and square root on the hardware instructions
-// (v_rcp_f32, v_sqrt_f32: 1 ulp).  Spelled as target builtins so that every
-// kernel instantiating a body, single or batched, selects the same
-// instructions: the lowering of a plain `/` or sqrtf depends on whether its
-// operands are wave-uniform, which differs between the two launch forms.
+// FP32 reciprocal, quotient, square root and reciprocal square root on the
+// hardware instructions (v_rcp_f32, v_sqrt_f32, v_rsq_f32: 1 ulp).  Spelled
+// as target builtins so that every kernel instantiating a body, single or
+// batched, selects the same instructions: the lowering of a plain `/` or
+// sqrtf depends on whether its operands are wave-uniform, which differs
+// between the two launch forms.
 __device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float div32(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
 __device__ __forceinline__ float sqrt32(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rsq32(float x) { return __builtin_amdgcn_rsqf(x); }
 
 template <int D>
 struct Prim32 {
@@ -583,8 +585,8 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
                 unP += minmod32<D>(sp.up[p], sn.up[p], gc, d, w, 1.0f) * S[p];
                 unN += minmod32<D>(sp.up[p], sn.up[p], gn, d, w, -1.0f) * S[p];
             }
-            const float cth0 = -div32(cdiv, sqrt32(cdiv * cdiv + ccurl + 4e-2f));
-            const float nth0 = -div32(ndiv, sqrt32(ndiv * ndiv + ncurl + 4e-2f));
+            const float cth0 = -cdiv * rsq32(cdiv * cdiv + ccurl + 4e-2f);
+            const float nth0 = -ndiv * rsq32(ndiv * ndiv + ncurl + 4e-2f);
             const float tha = 0.5f * (((cth0 < 0.f) ? 0.f : cth0) + ((nth0 < 0.f) ? 0.f : nth0));
             float gc[D], gn[D], hc[D], hn[D];
 #pragma unroll
@@ -599,9 +601,10 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
             const float pPd = minmod32<D>(sp.pp, sn.pp, hc, d, w, 1.0f);
             const float pNd = minmod32<D>(sp.pp, sn.pp, hn, d, w, -1.0f);
             const float unPf = phi_inf + unP, unNf = phi_inf + unN;
-            const float Msq = div32(unPf * unPf + unNf * unNf, 2.f * cavg * cavg * Sm2);
-            const float MP = div32(unPf, Sm * cavg);
-            const float MN = div32(unNf, Sm * cavg);
+            const float rSc = rcp32(Sm * cavg);
+            const float MP = unPf * rSc;
+            const float MN = unNf * rSc;
+            const float Msq = 0.5f * (MP * MP + MN * MN);
             const float Minf = 0.2f;
             const float mx = (Msq < Minf * Minf) ? Minf * Minf : Msq;
             const float M0 = sqrt32((mx < 1.0f) ? mx : 1.0f);
@@ -619,7 +622,7 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
                 const float s4 = MN * MN * MN * MN + MN * MN * MN * MP + MN * MN * MP * MP + MN * MP * MP * MP +
                                  MP * MP * MP * MP;
                 const float dd = 0.5f * (3.f - s2) + 2.f * alpha * (1.f - 2.f * s2 + s4);
-                dNP = div32(unN - unP, Sm * cavg) * dd;
+                dNP = (unN - unP) * rSc * dd;
             } else {
                 dNP = dN - dP;
             }
@@ -643,7 +646,7 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
     float dmag2 = 0.f;
 #pragma unroll
     for (int k = 0; k < D; k++) dmag2 += d[k] * d[k];
-    const float dmi = rcp32(sqrt32(dmag2));
+    const float dmi = rsq32(dmag2);
     float dn[D], du[D][D], dT[D], tau[D][D], tmc[D][D];
     float duP[D][D], duN[D][D], sUP[D], sUN[D];
 #pragma unroll
