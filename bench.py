@@ -171,6 +171,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline cores (default: up to 16)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=64,
+                    help="requests of the serving stream run after the timed region (0: none)")
+    ap.add_argument("--serve-steps", type=int, default=500)
     ap.add_argument("--fp64-only", action="store_true")
     args = ap.parse_args()
     if args.steps is None:
@@ -306,7 +309,31 @@ def main():
         out["fp32p"] = {"cell_updates_per_s": rate32, "ms_per_step": ms32 / args.steps,
                         "speedup_vs_fp64": rate32 / rate64, "error_vs_fp64": err,
                         "error_steps": args.warmup + args.steps + nprof}
+    if have32 and args.serve_requests > 0:
+        out["serving"] = serving_run(args.serve_requests, args.serve_steps)
     print(json.dumps(out))
+
+
+def serving_run(n_req, nsteps, budget_gib=2.0):
+    """A short serving stream after the timed region: seeded requests of
+    2D vortex / cylinder cases, 16k-1M cells, Poisson arrivals offering 80 %
+    of 1.5e9 cell-updates/s, FP32P, admitted under an HBM budget and stepped
+    in shared graph-replayed launches; time-to-solution p50 / p99."""
+    from lfm_public_amd import serving
+    print(f"[bench] serving: {n_req} requests x {nsteps} steps", file=sys.stderr, flush=True)
+    reqs = serving.make_stream(n_req, nsteps=nsteps)
+    srv = serving.Server(int(budget_gib * (1 << 30)))
+    try:
+        res = srv.run(reqs)
+    finally:
+        srv.close()
+    print(f"[bench] serving: p50 {res['tts_p50_s']:.3f} s, p99 {res['tts_p99_s']:.3f} s, "
+          f"{res['cell_updates_per_s']:.3e} cell-updates/s", file=sys.stderr, flush=True)
+    return {"requests": n_req, "steps_per_request": nsteps, "hbm_budget_gib": budget_gib,
+            "tts_p50_s": res["tts_p50_s"], "tts_p99_s": res["tts_p99_s"], "tts_max_s": res["tts_max_s"],
+            "cell_updates_per_s": res["cell_updates_per_s"], "wall_s": res["wall_s"],
+            "max_batch_members": res["max_batch_members"], "all_finite": res["all_finite"],
+            "arrivals": "Poisson at 80 % of 1.5e9 cell-updates/s"}
 
 
 if __name__ == "__main__":
