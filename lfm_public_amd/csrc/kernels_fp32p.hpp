@@ -1013,8 +1013,12 @@ template <int D, int GRAD, bool VISC, bool RG>
 __global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
     grad_vis32_body<D, GRAD, VISC, RG>(a, LFM_TID(blockIdx.x));
 }
+// 3D M1 / M2 faces: 5 waves per SIMD fit in 96 VGPRs without spills (97
+// would allow 4): O-grid face kernel 1.107 -> 1.033 ms per step.  Forcing
+// more waves on the update and gradient kernels was measured slower.
 template <int D, int SOLVER, bool VISC, bool STORED>
-__global__ void __launch_bounds__(256) face_flux32(KArgs32 a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 3 && SOLVER < 2 ? 5 : 1)))
+face_flux32(KArgs32 a) {
     face_flux32_body<D, SOLVER, VISC, STORED>(a, LFM_TID(blockIdx.x));
 }
 template <int D, bool RES, int CFL>
