@@ -4,9 +4,10 @@ update forms it equals the stand-alone compute_cfl / compute_dt of the same
 state bit for bit: both take the min / max over the same per-cell values,
 and min / max are exact in any order.  Covers FP64 and FP32P, 2D and 3D,
 the adjusted-dt mode, the one-stage scheme whose only stage also forms the
-residual (the reduction then keeps its own launch), partition groups and
-batched solvers.
+residual (the reduction then keeps its own launch), rkOrder 0 (no stage: the
+state stays, the reduction runs), partition groups and batched solvers.
 """
+import numpy as np
 import pytest
 
 import small_cases as sc
@@ -22,6 +23,9 @@ CASES = {
     "box_M1_adjust": lambda: sc.with_params(sc.PARITY_CASES["box_M1_visc"](), adjust_dt=1, max_co=0.3),
     "vortex_rk1_res_adjust": lambda: sc.vortex(16, solver=0, mu=0.02, rk_order=1, adjust_dt=1, max_co=0.2,
                                                have_res=1),
+    # rkOrder 0: Mesh::solve runs no stage (mesh_solver.cpp:500); the state
+    # stays and only the end-of-step reduction runs
+    "vortex_rk0": lambda: sc.vortex(8, solver=0, rk_order=0),
 }
 
 
@@ -40,10 +44,14 @@ def test_step_reduction_equals_standalone(name, prec):
     print(f"[cfl] {name} {prec}", flush=True)
     c = CASES[name]()
     g = capi.GpuSolver.from_case(c, precision=PRECS[prec])
+    q0 = g.q()
     for n in (1, 3):
         g.run(n)
         g.synchronize()
         _check_end_of_step(g, c)
+    if c.params["rk_order"] == 0:
+        assert np.array_equal(g.q(), q0)
+        assert g.info().steps_done == 4
     g.close()
 
 
