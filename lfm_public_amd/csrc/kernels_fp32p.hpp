@@ -270,21 +270,21 @@ __device__ __forceinline__ void vis_tensors32(const KArgs32 &a, const float (&du
     for (int i = 0; i < D; i++) sU[i] = dot32<D>(u, tau[i]);
 }
 
-// One cell's gradients as one record, padded to whole 16-byte words: dudx
+// One cell's gradients as one record, padded to whole 8-byte words: dudx
 // (D*D), dTdx (D), then with RG (the M2AUSM scheme) the rho and p gradients
 // (D each); ghosts' records stay zero.  A gathered neighbour's gradients are
-// a few vector loads.
+// a few 8-byte vector loads.
 template <int D, bool RG>
 struct GradRec32 {
     static constexpr int kDT = D * D;
     static constexpr int kRG = kDT + D;
     static constexpr int kPG = kRG + D;
     static constexpr int kN = RG ? kPG + D : kRG;
-    static constexpr int kStride = (kN + 3) / 4 * 4;
+    static constexpr int kStride = (kN + 1) / 2 * 2;
 };
 inline int grad_record_stride(int D, bool rg) {
     const int n = D * D + D + (rg ? 2 * D : 0);
-    return (n + 3) / 4 * 4;
+    return (n + 1) / 2 * 2;
 }
 
 // Green-Gauss sums of one cell over its faces in gather order; returns the
@@ -424,19 +424,24 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
         }
     }
     if (VISC) {
-        // the block's records are contiguous from its first cell: staged in
-        // LDS, then written as consecutive 16-byte words (without minmod the
-        // rho / p slots are written zero, as they stay)
-        constexpr int W = R::kStride / 4;
-        __shared__ float4 st[256 * W];
+        // the block's records are contiguous from its first cell (a multiple
+        // of 256, so 16-byte aligned): staged in LDS, then written as
+        // consecutive 16-byte words and a tail of single words (without
+        // minmod the rho / p slots are written zero, as they stay)
+        constexpr int S = R::kStride;
+        __shared__ __align__(16) float st[256 * S];
         const int t = threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < W; k++) st[t * W + k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+        for (int k = 0; k < S / 2; k++)
+            reinterpret_cast<float2 *>(st)[t * (S / 2) + k] = make_float2(r[2 * k], r[2 * k + 1]);
         __syncthreads();
         const int c0 = c - t;
         const int nrec = a.nc - c0 < (int)blockDim.x ? a.nc - c0 : (int)blockDim.x;
-        float4 *dst = reinterpret_cast<float4 *>(a.dudx + (size_t)c0 * R::kStride);
-        for (int j = t; j < nrec * W; j += blockDim.x) dst[j] = st[j];
+        float *dst = a.dudx + (size_t)c0 * S;
+        const int n = nrec * S, n4 = n / 4;
+        for (int j = t; j < n4; j += blockDim.x)
+            reinterpret_cast<float4 *>(dst)[j] = reinterpret_cast<const float4 *>(st)[j];
+        for (int j = 4 * n4 + t; j < n; j += blockDim.x) dst[j] = st[j];
     }
 }
 
@@ -450,14 +455,12 @@ struct GradGlobal32 {
     using R = GradRec32<D, RG>;
     // the gradient record of cell c
     __device__ __forceinline__ void rec(int c, float (&r)[R::kStride]) const {
-        const float4 *p = reinterpret_cast<const float4 *>(a.dudx + (size_t)c * R::kStride);
+        const float2 *p = reinterpret_cast<const float2 *>(a.dudx + (size_t)c * R::kStride);
 #pragma unroll
-        for (int k = 0; k < R::kStride / 4; k++) {
-            const float4 v = p[k];
-            r[4 * k] = v.x;
-            r[4 * k + 1] = v.y;
-            r[4 * k + 2] = v.z;
-            r[4 * k + 3] = v.w;
+        for (int k = 0; k < R::kStride / 2; k++) {
+            const float2 v = p[k];
+            r[2 * k] = v.x;
+            r[2 * k + 1] = v.y;
         }
     }
     // U_grad: dudx itself, or its own [component][ntot] array
