@@ -28,7 +28,8 @@ struct KArgs32 {
     int nc, nsm0, ng, nf, ntot;
     int ng_bnd;  // boundary ghosts [nc, nc+ng_bnd); halo ghosts after them
     const int32_t *fP, *fN;
-    const float *S, *d, *w;
+    const float *S;  // [nf][D]: a face's area vector as one record
+    const float *d, *w;
     const int32_t *g_off, *g_ent, *s_off, *s_ent;
     const int32_t *ghost_cell, *ghost_role;
     const float *vinv, *sigma;
@@ -67,6 +68,19 @@ __device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x
 __device__ __forceinline__ float div32(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
 __device__ __forceinline__ float sqrt32(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float rsq32(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// face f's area vector (one 8-byte load in 2D)
+template <int D>
+__device__ __forceinline__ void load_S32(const float *S, int f, float (&s)[D]) {
+    if constexpr (D == 2) {
+        const float2 v = reinterpret_cast<const float2 *>(S)[f];
+        s[0] = v.x;
+        s[1] = v.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < D; k++) s[k] = S[(size_t)f * D + k];
+    }
+}
 
 template <int D>
 struct Prim32 {
@@ -346,8 +360,7 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
         const int f = isP[j] ? (ent & kFaceMask) : ~ent;
         const int other = isP[j] ? a.fN[f] : a.fP[f];
         w[j] = a.w[f];
-#pragma unroll
-        for (int k = 0; k < D; k++) S[j][k] = a.S[(size_t)k * a.nf + f];
+        load_S32<D>(a.S, f, S[j]);
         load_q32<D>(a, other, qo[j]);
     }
 #pragma unroll
@@ -358,8 +371,7 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
         const bool p = ent >= 0;
         const int f = p ? (ent & kFaceMask) : ~ent;
         float Sf[D], q[D + 2];
-#pragma unroll
-        for (int k = 0; k < D; k++) Sf[k] = a.S[(size_t)k * a.nf + f];
+        load_S32<D>(a.S, f, Sf);
         load_q32<D>(a, p ? a.fN[f] : a.fP[f], q);
         add(p, a.w[f], Sf, q);
     }
@@ -487,11 +499,9 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
     const float w = a.w[f];
     const size_t nt = a.ntot;
     float S[D], d[D], qP[D + 2], qN[D + 2];
+    load_S32<D>(a.S, f, S);
 #pragma unroll
-    for (int k = 0; k < D; k++) {
-        S[k] = a.S[(size_t)k * a.nf + f];
-        d[k] = a.d[(size_t)k * a.nf + f];
-    }
+    for (int k = 0; k < D; k++) d[k] = a.d[(size_t)k * a.nf + f];
     load_q32<D>(a, P, qP);
     load_q32<D>(a, N, qN);
     const Prim32<D> sp = prim32<D>(a, qP), sn = prim32<D>(a, qN);
@@ -917,11 +927,10 @@ __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, cons
     for (int j = 0; j < U; j++) {
         const int ent = a.s_ent[e0 + (j < ne ? j : ne - 1)];
         const int f = ent >= 0 ? ent : ~ent;
+        float sf[D];
+        load_S32<D>(a.S, f, sf);
 #pragma unroll
-        for (int k = 0; k < D; k++) {
-            const double sv = (double)a.S[(size_t)k * a.nf + f];
-            Sj[j][k] = ent >= 0 ? sv : -sv;
-        }
+        for (int k = 0; k < D; k++) Sj[j][k] = ent >= 0 ? (double)sf[k] : -(double)sf[k];
     }
 #pragma unroll
     for (int j = 0; j < U; j++)
@@ -929,12 +938,11 @@ __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, cons
     for (int e = e0 + U; e < e0 + ne; e++) {
         const int ent = a.s_ent[e];
         const int f = ent >= 0 ? ent : ~ent;
+        float sf[D];
+        load_S32<D>(a.S, f, sf);
         double S[D];
 #pragma unroll
-        for (int k = 0; k < D; k++) {
-            const double sv = (double)a.S[(size_t)k * a.nf + f];
-            S[k] = ent >= 0 ? sv : -sv;
-        }
+        for (int k = 0; k < D; k++) S[k] = ent >= 0 ? (double)sf[k] : -(double)sf[k];
         add(S);
     }
     const double vol = a.vol_d[c];

@@ -24,7 +24,8 @@ struct KArgs {
     int nc, nsm0, ng, nf, ntot;
     int ng_bnd;  // boundary ghosts [nc, nc+ng_bnd); halo ghosts after them
     const int32_t *fP, *fN;
-    const double *S, *d, *w;        // S, d: [D][nf]
+    const double *S;                // [nf][D]: a face's area vector as one record
+    const double *d, *w;            // d: [D][nf]
     const int32_t *g_off, *g_ent;   // gather lists
     const int32_t *s_off, *s_ent;   // slot lists
     const int32_t *ghost_cell, *ghost_role;
@@ -87,6 +88,19 @@ template <int D>
 struct GatherUnroll {
     static constexpr int value = D == 2 ? 4 : 6;
 };
+
+// face f's area vector (one 16-byte load in 2D)
+template <int D>
+__device__ __forceinline__ void load_S64(const double *S, int f, double (&s)[D]) {
+    if constexpr (D == 2) {
+        const double2 v = reinterpret_cast<const double2 *>(S)[f];
+        s[0] = v.x;
+        s[1] = v.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < D; k++) s[k] = S[(size_t)f * D + k];
+    }
+}
 
 // State layout: components 0-3 of a cell (or ghost) i as one 32-byte
 // record, q[4 i + k], then component 4 (the 3D energy) as q[4 ntot + i]: a
@@ -422,8 +436,7 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
         isPj[j] = ent >= 0;
         const int f = isPj[j] ? (ent & kFaceMask) : ~ent;
         wj[j] = a.w[f];
-#pragma unroll
-        for (int k = 0; k < D; k++) Sj[j][k] = a.S[(size_t)k * a.nf + f];
+        load_S64<D>(a.S, f, Sj[j]);
         load_q<D>(a, isPj[j] ? a.fN[f] : a.fP[f], qj[j]);
     }
 #pragma unroll
@@ -434,8 +447,7 @@ __global__ void __launch_bounds__(256) grad_vis(KArgs a) {
         const bool p = ent >= 0;
         const int f = p ? (ent & kFaceMask) : ~ent;
         double Sf[D], q[D + 2];
-#pragma unroll
-        for (int k = 0; k < D; k++) Sf[k] = a.S[(size_t)k * a.nf + f];
+        load_S64<D>(a.S, f, Sf);
         load_q<D>(a, p ? a.fN[f] : a.fP[f], q);
         add(p, a.w[f], Sf, q);
     }
@@ -512,11 +524,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 2
     const double w = a.w[f];
     const size_t nt = a.ntot;
     double S[D], d[D], qP[D + 2], qN[D + 2];
+    load_S64<D>(a.S, f, S);
 #pragma unroll
-    for (int k = 0; k < D; k++) {
-        S[k] = a.S[(size_t)k * a.nf + f];
-        d[k] = a.d[(size_t)k * a.nf + f];
-    }
+    for (int k = 0; k < D; k++) d[k] = a.d[(size_t)k * a.nf + f];
     load_q<D>(a, P, qP);
     load_q<D>(a, N, qN);
     using R = GradRec64<D, SOLVER == 2>;
@@ -852,8 +862,10 @@ __device__ __forceinline__ double cfl_cell(const KArgs &a, const int c, const do
     for (int j = 0; j < U; j++) {
         const int ent = a.s_ent[e0 + (j < ne ? j : ne - 1)];
         const int f = ent >= 0 ? ent : ~ent;
+        double sf[D];
+        load_S64<D>(a.S, f, sf);
 #pragma unroll
-        for (int k = 0; k < D; k++) Sj[j][k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
+        for (int k = 0; k < D; k++) Sj[j][k] = ent >= 0 ? sf[k] : -sf[k];
     }
 #pragma unroll
     for (int j = 0; j < U; j++)
@@ -862,8 +874,10 @@ __device__ __forceinline__ double cfl_cell(const KArgs &a, const int c, const do
         const int ent = a.s_ent[e];
         const int f = ent >= 0 ? ent : ~ent;
         double S[D];
+        double sf[D];
+        load_S64<D>(a.S, f, sf);
 #pragma unroll
-        for (int k = 0; k < D; k++) S[k] = ent >= 0 ? a.S[(size_t)k * a.nf + f] : -a.S[(size_t)k * a.nf + f];
+        for (int k = 0; k < D; k++) S[k] = ent >= 0 ? sf[k] : -sf[k];
         add(S);
     }
     double v = MODE == 0 ? 100.0 : 0.0;

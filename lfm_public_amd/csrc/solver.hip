@@ -88,7 +88,13 @@ static int create_fp64(Solver *s) {
     int rc = 0;
     rc |= dev_upload(s, (int32_t **)&a.fP, L.fP.data(), nf);
     rc |= dev_upload(s, (int32_t **)&a.fN, L.fN.data(), nf);
-    rc |= dev_upload(s, (double **)&a.S, L.S.data(), (size_t)D * nf);
+    {
+        // S as one record per face ([nf][D])
+        std::vector<double> Sr((size_t)D * nf);
+        for (size_t f = 0; f < nf; f++)
+            for (int k = 0; k < D; k++) Sr[f * D + k] = L.S[(size_t)k * nf + f];
+        rc |= dev_upload(s, (double **)&a.S, Sr.data(), Sr.size());
+    }
     rc |= dev_upload(s, (double **)&a.d, L.d.data(), (size_t)D * nf);
     rc |= dev_upload(s, (double **)&a.w, L.w.data(), nf);
     rc |= dev_upload(s, (int32_t **)&a.g_off, L.g_off.data(), nc + 1);

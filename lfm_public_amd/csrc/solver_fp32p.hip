@@ -84,7 +84,11 @@ int create_fp32p(Solver *s) {
     rc |= up(s, (int32_t **)&a.fP, L.fP.data(), nf);
     rc |= up(s, (int32_t **)&a.fN, L.fN.data(), nf);
     {
-        auto S = to_f32(L.S), d = to_f32(L.d), w = to_f32(L.w);
+        // S as one record per face ([nf][D]); d and w per component
+        std::vector<float> S((size_t)D * nf);
+        for (size_t f = 0; f < nf; f++)
+            for (int k = 0; k < D; k++) S[f * D + k] = (float)L.S[(size_t)k * nf + f];
+        auto d = to_f32(L.d), w = to_f32(L.w);
         rc |= up(s, (float **)&a.S, S.data(), S.size());
         rc |= up(s, (float **)&a.d, d.data(), d.size());
         rc |= up(s, (float **)&a.w, w.data(), w.size());
