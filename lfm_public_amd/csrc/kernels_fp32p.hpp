@@ -840,7 +840,10 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
     for (int i = 0; i < ngf; i++) {
         const int g = a.fN[gface[i]] - a.nc;
         float qg[D + 2];
-        if (!ghost_state32<D>(a, a.ghost_role[g], q, qg)) continue;
+        if (!ghost_state32<D>(a, a.ghost_role[g], q, qg)) {
+            if (qout == a.q) continue;
+            load_q32<D>(a, a.nc + g, qg);  // an unset patch's ghost keeps its state
+        }
         store_q32<D>(qout, a.ntot, a.nc + g, qg);
     }
 }
@@ -851,9 +854,9 @@ __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, cons
 // every thread of the block must call this (block-wide reductions when RES
 // or CFL).  CFL >= 0 (the last stage of a step): the step's compute_dt (0) /
 // compute_cfl (1) of the updated cells, per-block extrema into a.partial.
-template <int D, bool RES, int CFL>
-__device__ __forceinline__ void cell_update32_body(const KArgs32 &a, float A, float B, int zero_dq, const int c,
-                                                   const int lblk, double cfl_param) {
+template <int D, bool RES, int CFL, class FA>
+__device__ __forceinline__ void cell_update32_body(const KArgs32 &a, const FA &fa, float *qout, float A, float B,
+                                                   int zero_dq, const int c, const int lblk, double cfl_param) {
     static_assert(!(RES && CFL >= 0), "the residual and the CFL share a.partial");
     __shared__ double red[256 / 64][D + 2];
     double rsq[D + 2];
@@ -862,7 +865,7 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, float A, fl
     for (int k = 0; k < D + 2; k++) rsq[k] = 0.0;
     if (c < a.nc) {
         float res[D + 2], q[D + 2];
-        cell_update32_cell<D, RES>(a, FluxGlobal32<D>{a}, A, B, zero_dq, c, a.q, res, q);
+        cell_update32_cell<D, RES>(a, fa, A, B, zero_dq, c, qout, res, q);
         if constexpr (CFL >= 0) cv = cfl_cell32<D, CFL>(a, c, q, cfl_param);
         if (RES) {
 #pragma unroll
@@ -1034,7 +1037,29 @@ face_flux32(KArgs32 a) {
 }
 template <int D, bool RES, int CFL>
 __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B, int zero_dq, double cfl_param) {
-    cell_update32_body<D, RES, CFL>(a, A, B, zero_dq, LFM_TID(blockIdx.x), LFM_BLK(), cfl_param);
+    cell_update32_body<D, RES, CFL>(a, FluxGlobal32<D>{a}, a.q, A, B, zero_dq, LFM_TID(blockIdx.x), LFM_BLK(),
+                                    cfl_param);
+}
+// Where a fused stage takes a face's flux: computed where it is gathered, by
+// the face kernel's own function, from the stage's input state a.q
+// (inviscid M1 / M2: no gradients).  Each interior face is computed by both
+// of its cells, to the same bits.
+template <int D, int SOLVER>
+struct FluxInline32 {
+    const KArgs32 &a;
+    __device__ __forceinline__ void get(int e, int f, float (&r)[D + 2]) const {
+        (void)e;
+        face_flux32_rhs<D, SOLVER, false, false>(a, f, GradGlobal32<D, false>{a}, 0, 0, r);
+    }
+};
+// One RK stage in one launch for inviscid M1 / M2: the face fluxes, the
+// ordered gather, sponge and update of cell_update32, reading the state in
+// a.q and writing the new state and boundary ghosts to qout (another buffer)
+template <int D, int SOLVER, int CFL>
+__global__ void __launch_bounds__(256) stage_fused32(KArgs32 a, float *qout, float A, float B, int zero_dq,
+                                                     double cfl_param) {
+    cell_update32_body<D, false, CFL>(a, FluxInline32<D, SOLVER>{a}, qout, A, B, zero_dq, LFM_TID(blockIdx.x),
+                                      LFM_BLK(), cfl_param);
 }
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
@@ -1101,7 +1126,8 @@ __global__ void __launch_bounds__(256) cell_update32_b(const KArgs32 *args, Batc
     int r, lb;
     batch_slot(m, r, lb);
     const KArgs32 &a = args[r];
-    cell_update32_body<D, false, CFL>(a, A, B, zero_dq, lb * 256 + threadIdx.x, lb, CFL >= 0 ? *a.dt : 0.0);
+    cell_update32_body<D, false, CFL>(a, FluxGlobal32<D>{a}, a.q, A, B, zero_dq, lb * 256 + threadIdx.x, lb,
+                                      CFL >= 0 ? *a.dt : 0.0);
 }
 // one block per solver: max of its partials into its CFL scalar
 static __global__ void __launch_bounds__(256) cfl_final32_b(const KArgs32 *args) {
@@ -1135,5 +1161,8 @@ void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool
                      const Solver::CflJob *cfl);
 template <int D>
 void launch_cfl_dt32(Solver *s, int mode, double param, double *out, double *dt_dst);
+template <int D>
+void launch_stage_fused32(Solver *s, double A, double B, bool zero_dq, const float *qin, float *qout,
+                          const Solver::CflJob *cfl);
 
 }  // namespace lfm

@@ -375,12 +375,41 @@ void Solver::end_step(bool own_dt) {
     if (prm.have_res) res_valid = true;
 }
 
+// the first stage run as one fused launch, nstages when none is: the
+// fused stages alternate between two buffers, so their count is even; the
+// residual stage keeps the separate launches (its per-block sums)
+int Solver::fused_from() const {
+    if (!q_alt32 || post_forces) return nstages;
+    int first = nstages % 2;
+    if (prm.have_res && first == 0) first = 2;
+    return first < nstages ? first : nstages;
+}
+
+void Solver::stage_fused(int rk, int first) {
+    stage_a(rk);  // the ghost fill after an outside write (no gradients here)
+    const bool even = (rk - first) % 2 == 0;
+    float *qin = even ? k32->q : q_alt32, *qout = even ? q_alt32 : k32->q;
+    prof_mark(3, true);
+    if (rk == nstages - 1) {
+        const CflJob job = cfl_job(true);
+        LFM_DISPATCH_D(this, launch_stage_fused32<2>(this, A[rk], B[rk], rk == 0, qin, qout, &job),
+                       launch_stage_fused32<3>(this, A[rk], B[rk], rk == 0, qin, qout, &job));
+        cfl_fused = true;
+    } else {
+        LFM_DISPATCH_D(this, launch_stage_fused32<2>(this, A[rk], B[rk], rk == 0, qin, qout, nullptr),
+                       launch_stage_fused32<3>(this, A[rk], B[rk], rk == 0, qin, qout, nullptr));
+    }
+    prof_mark(3, false);
+}
+
 // one full time step (Mesh::solve loop body), no host synchronisation
 void Solver::step() {
-    for (int rk = 0; rk < nstages; rk++) {
+    const int first = fused_from();
+    for (int rk = 0; rk < first; rk++) {
         stage_a(rk);
         stage_b(rk, true, true);
     }
+    for (int rk = first; rk < nstages; rk++) stage_fused(rk, first);
     end_step(true);
 }
 

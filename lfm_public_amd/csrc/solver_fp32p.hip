@@ -120,6 +120,10 @@ int create_fp32p(Solver *s) {
             for (size_t gi = 0; gi < (size_t)L.ng; gi++) q0[q32_index(k, nc + gi, ntot)] = (float)(-qi[k]);
         }
         rc |= up(s, &a.q, q0.data(), q0.size());
+        // the second state buffer of the fused stages (its unset-patch ghosts
+        // are copied from q by the first fused stage)
+        if (!s->visc && s->prm.solver < 2 && L.halo.empty())
+            rc |= up(s, &s->q_alt32, (float *)nullptr, q0.size());
     }
     rc |= up(s, &a.dq, (float *)nullptr, (size_t)(D + 2) * nc);
     rc |= up(s, &a.res_cell, (float *)nullptr, (size_t)(D + 2) * nc);
@@ -228,6 +232,40 @@ void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool
         hipLaunchKernelGGL((cell_update32<D, false, -1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
                            (float)B, (int)zero_dq, 0.0);
     }
+}
+
+template <int D, int SOLVER>
+static void launch_fused32_v(Solver *s, float A, float B, int zero_dq, const KArgs32 &a, float *qout,
+                             const Solver::CflJob *cfl) {
+    const int nb = nblocks(s->L.nc);
+    if (!cfl) {
+        hipLaunchKernelGGL((stage_fused32<D, SOLVER, -1>), dim3(nb), dim3(kBlock), 0, s->stream, a, qout, A, B,
+                           zero_dq, 0.0);
+        return;
+    }
+    if (cfl->mode == 0)
+        hipLaunchKernelGGL((stage_fused32<D, SOLVER, 0>), dim3(nb), dim3(kBlock), 0, s->stream, a, qout, A, B,
+                           zero_dq, cfl->param);
+    else
+        hipLaunchKernelGGL((stage_fused32<D, SOLVER, 1>), dim3(nb), dim3(kBlock), 0, s->stream, a, qout, A, B,
+                           zero_dq, cfl->param);
+    if (cfl->mode == 0)
+        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kFinalBlock), 0, s->stream, a.partial, nb, cfl->out,
+                           cfl->dt_dst);
+    else
+        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kFinalBlock), 0, s->stream, a.partial, nb, cfl->out,
+                           cfl->dt_dst);
+}
+
+template <int D>
+void launch_stage_fused32(Solver *s, double A, double B, bool zero_dq, const float *qin, float *qout,
+                          const Solver::CflJob *cfl) {
+    KArgs32 a = *s->k32;
+    a.q = const_cast<float *>(qin);
+    if (s->prm.solver == 0)
+        launch_fused32_v<D, 0>(s, (float)A, (float)B, (int)zero_dq, a, qout, cfl);
+    else
+        launch_fused32_v<D, 1>(s, (float)A, (float)B, (int)zero_dq, a, qout, cfl);
 }
 
 template <int D>
@@ -360,6 +398,10 @@ template void launch_flux32<2>(Solver *);
 template void launch_flux32<3>(Solver *);
 template void launch_update32<2>(Solver *, double, double, bool, bool, bool, const Solver::CflJob *);
 template void launch_update32<3>(Solver *, double, double, bool, bool, bool, const Solver::CflJob *);
+template void launch_stage_fused32<2>(Solver *, double, double, bool, const float *, float *,
+                                     const Solver::CflJob *);
+template void launch_stage_fused32<3>(Solver *, double, double, bool, const float *, float *,
+                                     const Solver::CflJob *);
 template void launch_cfl_dt32<2>(Solver *, int, double, double *, double *);
 template void launch_cfl_dt32<3>(Solver *, int, double, double *, double *);
 

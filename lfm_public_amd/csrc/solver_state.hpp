@@ -140,6 +140,12 @@ struct Solver {
     // then launches nothing.
     void stage_a(int rk);
     void stage_b(int rk, bool fuse_cfl = false, bool own_dt = true);
+    // FP32 inviscid M1 / M2 without halos or wall forces: stages from
+    // fused_from() on are one launch each, alternating between q and q_alt32
+    // (an even count, so a step ends in q)
+    float *q_alt32 = nullptr;
+    int fused_from() const;
+    void stage_fused(int rk, int first);
     void end_step(bool own_dt);
     void zero_state_for_step();
 
