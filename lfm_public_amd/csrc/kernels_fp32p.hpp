@@ -1055,8 +1055,10 @@ struct FluxInline32 {
 // One RK stage in one launch for inviscid M1 / M2: the face fluxes, the
 // ordered gather, sponge and update of cell_update32, reading the state in
 // a.q and writing the new state and boundary ghosts to qout (another buffer)
+// 3D: 4 waves per SIMD (127 instead of 130 VGPRs, no spills; 256^3 box
+// 5.25 -> 4.93 ms per step)
 template <int D, int SOLVER, int CFL>
-__global__ void __launch_bounds__(256) stage_fused32(KArgs32 a, float *qout, float A, float B, int zero_dq,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 3 ? 4 : 1))) stage_fused32(KArgs32 a, float *qout, float A, float B, int zero_dq,
                                                      double cfl_param) {
     cell_update32_body<D, false, CFL>(a, FluxInline32<D, SOLVER>{a}, qout, A, B, zero_dq, LFM_TID(blockIdx.x),
                                       LFM_BLK(), cfl_param);
