@@ -162,7 +162,11 @@ int lfm_one_rk_step(lfm_solver *s, int rk, double dt, double *res);
 int lfm_compute_dt(lfm_solver *s, double max_co, double *dt_out);
 int lfm_compute_cfl(lfm_solver *s, double dt, double *cfl_out);
 /* n iterations of the Mesh::solve loop body (mesh_solver.cpp:474-740):
- * prepare, all RK stages, then dt (adjustTimeStep) or CFL; no host sync. */
+ * prepare, all RK stages, then dt (adjustTimeStep) or CFL; no host sync.
+ * FP32P inviscid M1 / M2 solvers without halos or wall forces run most
+ * stages as one launch (face fluxes formed in the update, the state
+ * double-buffered); the results are the separate launches' bit for bit and
+ * the state is in the handle's buffer after every step. */
 int lfm_run(lfm_solver *s, int nsteps);
 /* Wait for all queued work of the handle. */
 int lfm_synchronize(lfm_solver *s);
@@ -174,7 +178,8 @@ int lfm_time_steps(lfm_solver *s, int nsteps, double *ms_out);
  * kernel group on the handle's stream: ms_per_group[5] = bc fill,
  * gradients+viscous, face fluxes, cell update (+ residual reduction; the
  * last stage's also forms the CFL/dt extrema and their final reduction
- * where fused), CFL/dt reduction; ms_total = whole. */
+ * where fused; the one-launch stages of lfm_run), CFL/dt reduction;
+ * ms_total = whole. */
 int lfm_profile_steps(lfm_solver *s, int nsteps, double *ms_per_group, double *ms_total);
 
 /* Fields in original cell order, SoA [comp][cell], FP64 on the host. */
