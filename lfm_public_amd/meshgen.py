@@ -438,12 +438,14 @@ def cell_centres_approx(m: PolyMesh) -> np.ndarray:
         idx = np.repeat(np.arange(nf), cnt)
         np.add.at(fc, idx, m.points[m.face_pts])
         fc /= cnt[:, None]
-    cc = np.zeros((m.n_cells, 3))
-    nfc = np.zeros(m.n_cells)
-    np.add.at(cc, m.owner, fc)
-    np.add.at(nfc, m.owner, 1)
-    np.add.at(cc, m.neighbour, fc[:m.n_internal])
-    np.add.at(nfc, m.neighbour, 1)
+    nc = m.n_cells
+    own = np.asarray(m.owner)
+    nei = np.asarray(m.neighbour)
+    ni = m.n_internal
+    nfc = np.bincount(own, minlength=nc) + np.bincount(nei, minlength=nc)
+    cc = np.empty((nc, 3))
+    for k in range(3):
+        cc[:, k] = np.bincount(own, fc[:, k], minlength=nc) + np.bincount(nei, fc[:ni, k], minlength=nc)
     return cc / nfc[:, None]
 
 

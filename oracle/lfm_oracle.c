@@ -1500,8 +1500,9 @@ int lfmo_ncomp(void *h, int which)
     int D = ((oracle *)h)->D;
     switch (which) {
     case LFMO_F_Q: case LFMO_F_DQ: case LFMO_F_RES: return D + 2;
-    case LFMO_F_DUDX: case LFMO_F_TAUMC: case LFMO_F_UGRAD: return D * D;
-    case LFMO_F_DTDX: case LFMO_F_SIGMAU: case LFMO_F_RHOGRAD: case LFMO_F_PGRAD: return D;
+    case LFMO_F_DUDX: case LFMO_F_TAUMC: case LFMO_F_UGRAD: case LFMO_F_RHOUGRAD: return D * D;
+    case LFMO_F_DTDX: case LFMO_F_SIGMAU: case LFMO_F_RHOGRAD: case LFMO_F_PGRAD:
+    case LFMO_F_RHOEGRAD: case LFMO_F_RPSIGRAD: case LFMO_F_CGRAD: return D;
     case LFMO_F_CENTRE: return 3;
     default: return 1;
     }
@@ -1533,12 +1534,28 @@ int lfmo_get_field(void *h, int which, double *out)
             case LFMO_F_CENTRE: x = o->centre[3 * c + k]; break;
             case LFMO_F_VOLUME: x = o->volume[c]; break;
             case LFMO_F_SUBMESH: x = v->sm_id; break;
+            case LFMO_F_RHOUGRAD: x = v->rhoU_grad[k / D][k % D]; break;
+            case LFMO_F_RHOEGRAD: x = v->rhoE_grad[k]; break;
+            case LFMO_F_RPSIGRAD: x = v->Rpsi_grad[k]; break;
+            case LFMO_F_CGRAD: x = v->c_grad[k]; break;
             default: return -1;
             }
             out[(size_t)k * nc + c] = x;
         }
     }
     return 0;
+}
+
+int lfmo_get_ghosts(void *h, double *q, int *cell, int *role)
+{
+    oracle *o = h;
+    for (int g = 0; g < o->ng; g++) {
+        if (q)
+            for (int k = 0; k < o->D + 2; k++) q[(size_t)k * o->ng + g] = o->v[o->nc + g].q[k];
+        if (cell) cell[g] = o->orig[o->ghost_cell[g]];
+        if (role) role[g] = o->ghost_role[g];
+    }
+    return o->ng;
 }
 
 int lfmo_get_slots(void *h, int *nface, int *nvalid, int *nbr, double *S, double *d, double *w)

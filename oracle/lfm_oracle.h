@@ -83,7 +83,13 @@ enum {
     LFMO_F_SIGMA = 11,  /* 1  sponge sigma                              */
     LFMO_F_CENTRE = 12, /* 3  cell centre                               */
     LFMO_F_VOLUME = 13, /* 1                                            */
-    LFMO_F_SUBMESH = 14 /* 1                                            */
+    LFMO_F_SUBMESH = 14, /* 1                                           */
+    /* gradients calc_gradients[_M2AUSM] forms that no flux consumes
+     * (cfd_v0.cpp:1384-1567); read by the oracle's own pins only */
+    LFMO_F_RHOUGRAD = 15, /* D*D, row-major [j][i] = d(rho U_j)/dx_i      */
+    LFMO_F_RHOEGRAD = 16, /* D                                          */
+    LFMO_F_RPSIGRAD = 17, /* D                                          */
+    LFMO_F_CGRAD = 18     /* D                                          */
 };
 
 void  *lfmo_create(const lfmo_mesh *mesh, const lfmo_params *prm,
@@ -106,6 +112,10 @@ double lfmo_compute_cfl(void *h, double dt);
 int    lfmo_get_slots(void *h, int *nface, int *nvalid, int *nbr,
                       double *S, double *d, double *w);
 int    lfmo_n_ghost(void *h);
+/* Boundary ghosts as set_boundary_conditions last left them: q
+ * [(D+2)][ng], the original id of the interior cell each belongs to, and
+ * its patch role (LFMO_BC_*).  Any array may be NULL. */
+int    lfmo_get_ghosts(void *h, double *q, int *cell, int *role);
 
 /* Partitions (src/cfd_v0.cpp:432-748 MPI ghosts and send lists;
  * src/mesh_reader.cpp:390-612 neighbour centres; src/cfd_v0.cpp:3284-3692

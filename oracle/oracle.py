@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblfm_oracle.so")
 
 F_Q, F_DQ, F_DUDX, F_DTDX, F_TAUMC, F_SIGMAU, F_RHOGRAD, F_PGRAD, F_UGRAD, F_RES, F_VOLINV, F_SIGMA, \
-    F_CENTRE, F_VOLUME, F_SUBMESH = range(15)
+    F_CENTRE, F_VOLUME, F_SUBMESH, F_RHOUGRAD, F_RHOEGRAD, F_RPSIGRAD, F_CGRAD = range(19)
 
 
 class _Mesh(C.Structure):
@@ -71,6 +71,7 @@ def lib():
         L.lfmo_compute_cfl.argtypes = [C.c_void_p, C.c_double]
         L.lfmo_get_slots.argtypes = [C.c_void_p] + [C.c_void_p] * 6
         L.lfmo_n_ghost.argtypes = [C.c_void_p]
+        L.lfmo_get_ghosts.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.lfmo_minmod.restype = C.c_double
         L.lfmo_minmod.argtypes = [C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_double, C.c_double]
         L.lfmo_p5.argtypes = [C.c_double, C.c_double, C.c_void_p]
@@ -239,6 +240,15 @@ class Oracle:
     @property
     def n_ghost(self) -> int:
         return lib().lfmo_n_ghost(self.h)
+
+    def ghosts(self):
+        """Boundary ghosts: (q [(D+2), ng], interior cell id [ng], role [ng])."""
+        ng = self.n_ghost
+        q = np.zeros((self.dim + 2, max(ng, 1)))
+        cell = np.zeros(max(ng, 1), np.int32)
+        role = np.zeros(max(ng, 1), np.int32)
+        lib().lfmo_get_ghosts(self.h, _ptr(q), _ptr(cell), _ptr(role))
+        return q[:, :ng], cell[:ng], role[:ng]
 
     # post-processing ---------------------------------------------------------
     def set_boundary_conditions(self):
