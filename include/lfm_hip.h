@@ -175,10 +175,11 @@ int lfm_synchronize(lfm_solver *s);
 int lfm_time_steps(lfm_solver *s, int nsteps, double *ms_out);
 
 /* Time n steps launched kernel by kernel with HIP events around every
- * kernel group on the handle's stream: ms_per_group[5] = bc fill,
+ * kernel group on the handle's stream: ms_per_group[6] = bc fill,
  * gradients+viscous, face fluxes, cell update (+ residual reduction; the
- * last stage's also forms the CFL/dt extrema and their final reduction
- * where fused; the one-launch stages of lfm_run), CFL/dt reduction;
+ * one-launch stages of lfm_run), CFL/dt (the per-cell launch where it has
+ * its own, and every final single-block reduction), the last stage's
+ * update (or one-launch stage) that also forms the CFL/dt block extrema;
  * ms_total = whole. */
 int lfm_profile_steps(lfm_solver *s, int nsteps, double *ms_per_group, double *ms_total);
 

@@ -291,14 +291,14 @@ class GpuSolver:
         _check(load().lfm_time_steps(self.h, nsteps, C.byref(ms)))
         return ms.value
 
-    PROFILE_GROUPS = ("bc_fill", "grad_vis", "face_flux", "cell_update", "cfl_dt")
+    PROFILE_GROUPS = ("bc_fill", "grad_vis", "face_flux", "cell_update", "cfl_dt", "cell_update_cfl")
 
     def profile_steps(self, nsteps: int):
         """(ms per kernel group summed over nsteps, total ms) from HIP events."""
         per = np.zeros(8)
         tot = C.c_double()
         _check(load().lfm_profile_steps(self.h, nsteps, _ptr(per), C.byref(tot)))
-        return dict(zip(self.PROFILE_GROUPS, per[:5].tolist())), tot.value
+        return dict(zip(self.PROFILE_GROUPS, per[:6].tolist())), tot.value
 
     def info(self) -> Info:
         i = Info()

@@ -643,7 +643,7 @@ struct lfm_case {
     // fields
     std::vector<double> p, T, U, alpha;
     bool have_alpha = false;
-    Field fp, fT, fU;
+    Field fp, fT, fU, falpha;
     // controls
     lfm_params prm{};
     double start = 0, end = 0, dt = 0, write_interval = 1;
@@ -772,6 +772,26 @@ static void read_mesh(lfm_case &c) {
     }
 }
 
+// the smallest (latest = false) or largest numeric time directory of a case
+// directory; "constant" and other names are not times
+static double time_dir_extreme(const std::string &dir, bool latest) {
+    DIR *d = opendir(dir.c_str());
+    if (!d) perr(dir, "cannot list");
+    bool any = false;
+    double best = 0.0;
+    while (dirent *e = readdir(d)) {
+        const char *n = e->d_name;
+        char *end = nullptr;
+        const double v = std::strtod(n, &end);
+        if (end == n || *end || !std::isfinite(v) || !is_dir(dir + "/" + n)) continue;
+        if (!any || (latest ? v > best : v < best)) best = v;
+        any = true;
+    }
+    closedir(d);
+    if (!any) perr(dir, "no time directory to start from");
+    return best;
+}
+
 static void read_controls(lfm_case &c) {
     const std::string r = c.root + "/";
     DictFile control = load_dict(r + "system/controlDict");
@@ -803,8 +823,16 @@ static void read_controls(lfm_case &c) {
     p.mu = dict_scalar(thermo, "mixture/transport", "mu", 0, false);
     p.pr = dict_scalar(thermo, "mixture/transport", "Pr", 0, false);
     p.laminar = dict_word(turb, "", "simulationType", "", false) == "laminar";
-    // Foam::Time controls (runTimeManagerOF.cpp)
-    c.start = dict_scalar(control, "", "startTime", 0, true);
+    // Foam::Time controls (runTimeManagerOF.cpp): startFrom picks the start
+    // time as Foam::Time does -- the startTime entry (the default), or the
+    // first / latest time directory of the case (a restart)
+    const std::string from = dict_word(control, "", "startFrom", "startTime", true);
+    if (from == "startTime")
+        c.start = dict_scalar(control, "", "startTime", 0, true);
+    else if (from == "firstTime" || from == "latestTime")
+        c.start = time_dir_extreme(c.dir, from == "latestTime");
+    else
+        perr(r + "system/controlDict", "startFrom must be startTime, firstTime or latestTime, not " + from);
     c.end = dict_scalar(control, "", "endTime", 0, false);
     c.dt = dict_scalar(control, "", "deltaT", 0, false);
     p.dt = c.dt;
@@ -844,7 +872,10 @@ static void read_fields(lfm_case &c) {
     c.U = c.fU.v;
     struct stat st;
     c.have_alpha = stat((t + "alpha").c_str(), &st) == 0;
-    if (c.have_alpha) c.alpha = read_field(t + "alpha", n, 1).v;
+    if (c.have_alpha) {
+        c.falpha = read_field(t + "alpha", n, 1);
+        c.alpha = c.falpha.v;
+    }
 }
 
 static void write_field(const std::string &path, const char *cls, const char *name, const std::string &dims,
@@ -1042,6 +1073,12 @@ int lfm_case_write_time(lfm_case *c, double time, const double *q, int32_t dim) 
         const std::string bnd = calculated_boundary(*c);
         write_field(d + "/rho", "volScalarField", "rho", "[1 -3 0 0 0 0 0]", rho, 1, bnd);
         write_field(d + "/E", "volScalarField", "E", "[0 2 -2 0 0 0 0]", E, 1, bnd);
+        // the sponge distance is read with AUTO_WRITE and so lands in every
+        // written time directory (polyMeshReaderOF.cpp:700-709), where a
+        // restart from that time reads it again
+        if (c->have_alpha)
+            write_field(d + "/alpha", "volScalarField", "alpha", c->falpha.dimensions, c->alpha, 1,
+                        boundary_ascii(c->falpha));
     } catch (const ParseError &e) {
         return fail(LFM_EINVAL, e.msg);
     } catch (const std::exception &e) {

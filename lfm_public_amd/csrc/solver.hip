@@ -219,13 +219,9 @@ static void launch_update64(Solver *s, double A, double B, bool zero_dq, bool re
     } else if (cfl && cfl->mode == 0) {
         hipLaunchKernelGGL((cell_update<D, false, 0>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, A, B,
                            (int)zero_dq, cfl->param);
-        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k64.partial, nb, cfl->out,
-                           cfl->dt_dst);
     } else if (cfl) {
         hipLaunchKernelGGL((cell_update<D, false, 1>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, A, B,
                            (int)zero_dq, cfl->param);
-        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k64.partial, nb, cfl->out,
-                           cfl->dt_dst);
     } else {
         hipLaunchKernelGGL((cell_update<D, false, -1>), dim3(nb), dim3(kBlock), 0, s->stream, s->k64, A, B,
                            (int)zero_dq, 0.0);
@@ -301,6 +297,18 @@ void Solver::update(double A, double B, bool zero_dq, bool res, bool accumulate_
     }
 }
 
+// the single-block final reduction of the block extrema an update with a
+// CFL job left in the reduction scratch (shared by both precision paths)
+void Solver::cfl_final(const CflJob &job) {
+    const int nb = nblocks(L.nc);
+    if (job.mode == 0)
+        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kFinalBlock), 0, stream, k64.partial, nb, job.out,
+                           job.dt_dst);
+    else
+        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kFinalBlock), 0, stream, k64.partial, nb, job.out,
+                           job.dt_dst);
+}
+
 void Solver::cfl_dt(int mode, double param, double *out, double *dt_dst) {
     if (fp32) {
         LFM_DISPATCH_D(this, launch_cfl_dt32<2>(this, mode, param, out, dt_dst),
@@ -347,19 +355,24 @@ void Solver::stage_b(int rk, bool fuse_cfl, bool own_dt) {
     // the update refreshes the boundary ghosts from the updated cells; the
     // wall forces after this step read them as this stage began
     if (post_forces && rk == nstages - 1) post_snapshot();
-    prof_mark(3, true);
     // fused where measured faster: the FP64 3D update runs at 4 waves per
     // SIMD, where the appended slot gathers cost more than their own launch
     // (4.2 M-cell O-grid: 1.760 -> 1.779 ms per step; 2D and FP32 gain
     // 7-11 us per step on 1 M cells)
     if (fuse_cfl && rk == nstages - 1 && !res && (fp32 || L.D == 2)) {
         const CflJob job = cfl_job(own_dt);
+        prof_mark(5, true);
         update(A[rk], B[rk], rk == 0, false, false, &job);
+        prof_mark(5, false);
+        prof_mark(4, true);
+        cfl_final(job);
+        prof_mark(4, false);
         cfl_fused = true;
     } else {
+        prof_mark(3, true);
         update(A[rk], B[rk], rk == 0, res, false);
+        prof_mark(3, false);
     }
-    prof_mark(3, false);
 }
 
 void Solver::end_step(bool own_dt) {
@@ -375,41 +388,73 @@ void Solver::end_step(bool own_dt) {
     if (prm.have_res) res_valid = true;
 }
 
-// the first stage run as one fused launch, nstages when none is: the
-// fused stages alternate between two buffers, so their count is even; the
-// residual stage keeps the separate launches (its per-block sums)
+// the second state buffer of the one-launch stages, allocated on first use
+// (under the set-up mutex, as a capture on another thread requires); when
+// it cannot be had the step runs the separate launches, which compute the
+// same bits
+bool Solver::ensure_alt32() {
+    if (!fused_ok) return false;
+    if (q_alt32) return true;
+    const size_t n = (size_t)(L.D + 2) * ((size_t)L.nc + L.ng);
+    std::lock_guard<std::mutex> lock(device_setup_mutex());
+    void *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        fused_ok = false;
+        return false;
+    }
+    q_alt32 = (float *)p;
+    allocs.push_back(p);
+    device_bytes += (int64_t)(n * sizeof(float));
+    return true;
+}
+
+// the first stage run as one fused launch (nstages when none is): every
+// stage, except a residual stage 0, which keeps the separate launches for
+// its per-block sums
 int Solver::fused_from() const {
-    if (!q_alt32 || post_forces) return nstages;
-    int first = nstages % 2;
-    if (prm.have_res && first == 0) first = 2;
+    if (!fused_ok || post_forces) return nstages;
+    const int first = prm.have_res ? 1 : 0;
     return first < nstages ? first : nstages;
 }
 
-void Solver::stage_fused(int rk, int first) {
+void Solver::stage_fused(int rk, bool from_alt) {
     stage_a(rk);  // the ghost fill after an outside write (no gradients here)
-    const bool even = (rk - first) % 2 == 0;
-    float *qin = even ? k32->q : q_alt32, *qout = even ? q_alt32 : k32->q;
-    prof_mark(3, true);
+    float *qin = from_alt ? q_alt32 : k32->q, *qout = from_alt ? k32->q : q_alt32;
     if (rk == nstages - 1) {
         const CflJob job = cfl_job(true);
+        prof_mark(5, true);
         LFM_DISPATCH_D(this, launch_stage_fused32<2>(this, A[rk], B[rk], rk == 0, qin, qout, &job),
                        launch_stage_fused32<3>(this, A[rk], B[rk], rk == 0, qin, qout, &job));
+        prof_mark(5, false);
+        prof_mark(4, true);
+        cfl_final(job);
+        prof_mark(4, false);
         cfl_fused = true;
     } else {
+        prof_mark(3, true);
         LFM_DISPATCH_D(this, launch_stage_fused32<2>(this, A[rk], B[rk], rk == 0, qin, qout, nullptr),
                        launch_stage_fused32<3>(this, A[rk], B[rk], rk == 0, qin, qout, nullptr));
+        prof_mark(3, false);
     }
-    prof_mark(3, false);
 }
 
 // one full time step (Mesh::solve loop body), no host synchronisation
 void Solver::step() {
-    const int first = fused_from();
+    const int first = fused_ok && ensure_alt32() ? fused_from() : nstages;
     for (int rk = 0; rk < first; rk++) {
         stage_a(rk);
         stage_b(rk, true, true);
     }
-    for (int rk = first; rk < nstages; rk++) stage_fused(rk, first);
+    bool in_alt = false;
+    for (int rk = first; rk < nstages; rk++) {
+        stage_fused(rk, in_alt);
+        in_alt = !in_alt;
+    }
+    // an odd number of one-launch stages left the state in the second
+    // buffer: it becomes the state buffer (kernels take k32 by value at
+    // launch, so later launches see the swap)
+    if (in_alt) std::swap(k32->q, q_alt32);
     end_step(true);
 }
 

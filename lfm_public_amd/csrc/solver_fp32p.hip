@@ -120,10 +120,10 @@ int create_fp32p(Solver *s) {
             for (size_t gi = 0; gi < (size_t)L.ng; gi++) q0[q32_index(k, nc + gi, ntot)] = (float)(-qi[k]);
         }
         rc |= up(s, &a.q, q0.data(), q0.size());
-        // the second state buffer of the fused stages (its unset-patch ghosts
-        // are copied from q by the first fused stage)
-        if (!s->visc && s->prm.solver < 2 && L.halo.empty())
-            rc |= up(s, &s->q_alt32, (float *)nullptr, q0.size());
+        // one-launch stages (a second state buffer, allocated on the first
+        // step that uses it; its unset-patch ghosts are copied from q by the
+        // first fused stage)
+        s->fused_ok = !s->visc && s->prm.solver < 2 && L.halo.empty();
     }
     rc |= up(s, &a.dq, (float *)nullptr, (size_t)(D + 2) * nc);
     rc |= up(s, &a.res_cell, (float *)nullptr, (size_t)(D + 2) * nc);
@@ -221,13 +221,9 @@ void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool
     } else if (cfl && cfl->mode == 0) {
         hipLaunchKernelGGL((cell_update32<D, false, 0>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
                            (float)B, (int)zero_dq, cfl->param);
-        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k32->partial, nb, cfl->out,
-                           cfl->dt_dst);
     } else if (cfl) {
         hipLaunchKernelGGL((cell_update32<D, false, 1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
                            (float)B, (int)zero_dq, cfl->param);
-        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kFinalBlock), 0, s->stream, s->k32->partial, nb, cfl->out,
-                           cfl->dt_dst);
     } else {
         hipLaunchKernelGGL((cell_update32<D, false, -1>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32, (float)A,
                            (float)B, (int)zero_dq, 0.0);
@@ -249,12 +245,6 @@ static void launch_fused32_v(Solver *s, float A, float B, int zero_dq, const KAr
     else
         hipLaunchKernelGGL((stage_fused32<D, SOLVER, 1>), dim3(nb), dim3(kBlock), 0, s->stream, a, qout, A, B,
                            zero_dq, cfl->param);
-    if (cfl->mode == 0)
-        hipLaunchKernelGGL(cfl_dt_final<0>, dim3(1), dim3(kFinalBlock), 0, s->stream, a.partial, nb, cfl->out,
-                           cfl->dt_dst);
-    else
-        hipLaunchKernelGGL(cfl_dt_final<1>, dim3(1), dim3(kFinalBlock), 0, s->stream, a.partial, nb, cfl->out,
-                           cfl->dt_dst);
 }
 
 template <int D>

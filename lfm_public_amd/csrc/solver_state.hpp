@@ -63,7 +63,9 @@ struct Solver {
     KArgs32 *k32 = nullptr;       // owned via allocs-free host block
 
     // per-kernel-group timing with HIP events (lfm_profile_steps)
-    static constexpr int kProfGroups = 5;  // bc, grad_vis, face_flux, cell_update, cfl_dt
+    // bc, grad_vis, face_flux, cell_update (and one-launch stages), cfl_dt
+    // (final reductions), the last stage's update forming the CFL/dt extrema
+    static constexpr int kProfGroups = 6;
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;       // pairs (start, stop)
     std::vector<int> prof_grp;
@@ -131,6 +133,9 @@ struct Solver {
     // reads only a cell's own state and its faces' areas); not with res
     void update(double A, double B, bool zero_dq, bool res, bool accumulate_res, const CflJob *cfl = nullptr);
     void cfl_dt(int mode, double param, double *out, double *dt_dst);
+    // the final single-block reduction after an update that formed the
+    // block extrema of a CFL job
+    void cfl_final(const CflJob &job);
     void step();
     // pieces of step(): stage part A (ghost fill if stale, gradients and
     // viscous tensors), part B (fluxes, update), and the end-of-step CFL/dt
@@ -140,12 +145,17 @@ struct Solver {
     // then launches nothing.
     void stage_a(int rk);
     void stage_b(int rk, bool fuse_cfl = false, bool own_dt = true);
-    // FP32 inviscid M1 / M2 without halos or wall forces: stages from
-    // fused_from() on are one launch each, alternating between q and q_alt32
-    // (an even count, so a step ends in q)
+    // FP32 inviscid M1 / M2 without halos or wall forces (fused_ok): stages
+    // from fused_from() on are one launch each, reading one state buffer and
+    // writing the other; after an odd count the two buffers swap roles, so
+    // k32->q always holds the state between steps.  The second buffer is
+    // allocated by the first step that uses it (ensure_alt32), so solvers
+    // that only step through a batch never hold it.
+    bool fused_ok = false;
     float *q_alt32 = nullptr;
+    bool ensure_alt32();
     int fused_from() const;
-    void stage_fused(int rk, int first);
+    void stage_fused(int rk, bool from_alt);
     void end_step(bool own_dt);
     void zero_state_for_step();
 

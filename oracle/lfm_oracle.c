@@ -1546,6 +1546,21 @@ int lfmo_get_field(void *h, int which, double *out)
     return 0;
 }
 
+int lfmo_set_field(void *h, int which, const double *in)
+{
+    oracle *o = h;
+    const int nc = o->nc;
+    if (which != LFMO_F_Q && which != LFMO_F_DQ) return -1;
+    for (int ci = 0; ci < nc; ci++) {
+        int c = o->orig[ci];
+        for (int k = 0; k < o->D + 2; k++) {
+            double x = in[(size_t)k * nc + c];
+            if (which == LFMO_F_Q) o->v[ci].q[k] = x; else o->v[ci].dq[k] = x;
+        }
+    }
+    return 0;
+}
+
 int lfmo_get_ghosts(void *h, double *q, int *cell, int *role)
 {
     oracle *o = h;

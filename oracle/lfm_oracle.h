@@ -101,6 +101,9 @@ int    lfmo_run(void *h, int nsteps);
 /* One stage only (for per-stage parity); begin_step performs prepare_for_timestep. */
 int    lfmo_stage(void *h, int rk, int begin_step);
 int    lfmo_get_field(void *h, int which, double *out);
+/* Overwrite the cells' q or dq (LFMO_F_Q / LFMO_F_DQ), original cell order;
+ * ghosts follow at the next boundary fill. */
+int    lfmo_set_field(void *h, int which, const double *in);
 int    lfmo_ncomp(void *h, int which);
 double lfmo_get_dt(void *h);
 double lfmo_get_cfl(void *h);

@@ -59,6 +59,7 @@ def lib():
         L.lfmo_run.argtypes = [C.c_void_p, C.c_int]
         L.lfmo_stage.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.lfmo_get_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.lfmo_set_field.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.lfmo_ncomp.argtypes = [C.c_void_p, C.c_int]
         L.lfmo_get_dt.restype = C.c_double
         L.lfmo_get_dt.argtypes = [C.c_void_p]
@@ -206,6 +207,13 @@ class Oracle:
 
     def q(self) -> np.ndarray:
         return self.field(F_Q)
+
+    def set_field(self, which: int, values: np.ndarray):
+        """Overwrite q (F_Q) or dq (F_DQ), [(D+2), cells] in original order."""
+        v = np.ascontiguousarray(values, np.float64)
+        assert v.shape == (self.dim + 2, self.n_cells)
+        if lib().lfmo_set_field(self.h, which, _ptr(v)) != 0:
+            raise ValueError("only q and dq can be set")
 
     @property
     def dt(self) -> float:

@@ -67,6 +67,10 @@ PARITY_CASES = {
     "cylinder_M2_minmod": lambda: cylinder(48, solver=1, minmod=1),
     "box_M1_visc": lambda: seeded_perturbation(box(6, solver=0, mu=1e-3)),
     "box_M2AUSM_visc": lambda: with_params(seeded_perturbation(box(6, solver=2, mu=1e-3)), minmod=1),
+    # inviscid 3D (the 256^3 acoustic box runs M1 with mu = 0): the mu = 0
+    # kernel instantiations, and in FP32 the one-launch stage
+    "box_M1_euler": lambda: seeded_perturbation(box(6, solver=0)),
+    "box_M2_euler_rk4": lambda: seeded_perturbation(box(6, solver=1, rk_order=4)),
     "ogrid_M2AUSM_cyclic": lambda: ogrid(6, 12, 4, solver=2, minmod=1),
     "ogrid_M1_cyclic_rk2": lambda: ogrid(6, 12, 4, solver=0, minmod=0, rk_order=2),
     # simulationType LES: calc_VIS_Smagorinsky

@@ -177,6 +177,49 @@ def test_write_time_round_trip(tmp_path):
     fr.close()
 
 
+def test_start_from_latest_and_first_time(tmp_path):
+    """controlDict startFrom as Foam::Time reads it: startTime (the entry),
+    firstTime (the smallest time directory), latestTime (the largest, a
+    restart; its sponge distance alpha is read there too, as the writer puts
+    it there); any other word is refused."""
+    c = sc.PARITY_CASES["cylinder_M2AUSM_sponge"]()
+    d = str(tmp_path / "r")
+    foam.write_case(d, c, end_steps=10)
+    fc = foam.FoamCase(d)
+    o = O.Oracle(c.mesh, c.params, c.p, c.T, c.U, c.alpha)
+    o.run(3)
+    t3 = 3 * c.params["dt"]
+    fc.write_time(t3, o.q(), c.params["dim"])
+    o.run(2)
+    t5 = 5 * c.params["dt"]
+    fc.write_time(t5, o.q(), c.params["dim"])
+    fc.close()
+    assert os.path.isfile(os.path.join(d, "%.12g" % t5, "alpha"))
+    os.makedirs(os.path.join(d, "constant", "notatime"), exist_ok=True)
+    cd = os.path.join(d, "system", "controlDict")
+    base = open(cd).read()
+    assert "startFrom       startTime;" in base
+
+    def reopen(word):
+        open(cd, "w").write(base.replace("startFrom       startTime;", f"startFrom       {word};"))
+        return foam.FoamCase(d)
+
+    fl = reopen("latestTime")
+    assert fl.start == pytest.approx(t5, rel=1e-12)
+    g, R = cases.gamma_of(c.params), cases.gas_constant(c.params["mol_weight"])
+    q = o.q()
+    U = q[1:3] / q[0]
+    p = q[0] * (g - 1) * (q[3] / q[0] - 0.5 * (U * U).sum(0))
+    assert np.array_equal(fl.p, p)
+    assert np.array_equal(fl.alpha, c.alpha)
+    fl.close()
+    ff = reopen("firstTime")
+    assert ff.start == 0.0 and np.array_equal(ff.p, c.p)
+    ff.close()
+    with pytest.raises(capi.LfmError, match="startFrom"):
+        reopen("nextWrite")
+
+
 @pytest.mark.parametrize("name", ["cylinder_M2AUSM_sponge", "ogrid_M1_cyclic_rk2"])
 def test_binary_case_reads_like_ascii(tmp_path, name):
     """A case written in OpenFOAM's binary format (raw point, face, label

@@ -1,0 +1,125 @@
+"""The 3D acoustic box of the 256^3 benchmark configuration (needs an MI355X).
+
+Configuration: unit-spaced hex box with inlet/outlet patches on all six
+sides, uniform flow at M = 0.1, a Gaussian pressure pulse of 1e-3 p_inf
+(isentropic density), M1 (central-upwind) flux, mu = 0, RK5 at CFL 0.5.
+Inviscid M1 runs the mu = 0 kernel instantiations: in FP64 the face kernel
+face_flux<3,0,false,false> and the update; in FP32 the one-launch stage
+stage_fused32<3,0,*> (fluxes formed in the update's ordered gather).
+
+* 24^3 (pulse half-width scaled to 3 cells): FP64 against the oracle after
+  5 steps, bit for bit (the bar the tests write is 1e-10 relative L2 on p,
+  U, T; the FP64 kernels keep the reference's operation order).
+* 24^3: FP32 deviation path against FP64 after 200 steps, perturbation-
+  normalised relative L2 <= 1e-5 on p, T and each velocity component.
+* 256^3 (16,777,216 cells), the benchmark size, 10 steps, through properties
+  that do not need the oracle: every value finite; the FP64 conservation
+  integrals (mass, momentum, energy) unchanged to round-off while the pulse
+  is still far inside the box (the boundary states stay the freestream, so
+  inflow and outflow fluxes cancel); FP32 against FP64 on p', T and U within
+  the same 1e-5 bar, and its integrals equal FP64's to FP32 round-off.
+"""
+import numpy as np
+import pytest
+
+from conftest import pert_l2, primitives, rel_l2
+from lfm_public_amd import capi, cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _box(n, halfwidth):
+    c = cases.acoustic_box(n=n, halfwidth_cells=halfwidth)
+    assert c.params["solver"] == 0 and c.params["mu"] == 0.0 and c.params["rk_order"] == 5
+    assert c.params["mach"] == 0.1
+    return c
+
+
+def _prims(case, q):
+    g, R = cases.gamma_of(case.params), cases.gas_constant(case.params["mol_weight"])
+    return primitives(q, 3, g, R)
+
+
+def _pert_errors(case, q32, q64):
+    p3, U3, T3 = _prims(case, q32)
+    p6, U6, T6 = _prims(case, q64)
+    prm = case.params
+    errs = {"p": pert_l2(p3, p6, prm["p_inf"]), "T": pert_l2(T3, T6, prm["T_inf"])}
+    for k in range(3):
+        errs[f"U{k}"] = pert_l2(U3[k], U6[k], prm["U_inf"][k])
+    return errs
+
+
+def test_box24_fp64_matches_oracle(oracle_lib):
+    import oracle as O
+    case = _box(24, 3.0)
+    print(f"[box] 24^3 FP64 vs oracle, {case.n_cells} cells, 5 steps", flush=True)
+    g = capi.GpuSolver.from_case(case)
+    o = O.Oracle(case.mesh, case.params, case.p, case.T, case.U, case.alpha)
+    q_init = o.q()
+    assert np.array_equal(g.q(), q_init)
+    g.run(5)
+    o.run(5)
+    qg, qo = g.q(), o.q()
+    g.close()
+    assert np.all(np.isfinite(qg))
+    pg, Ug, Tg = _prims(case, qg)
+    po, Uo, To = _prims(case, qo)
+    errs = {"p": rel_l2(pg, po), "T": rel_l2(Tg, To), "U": rel_l2(Ug, Uo)}
+    print({k: f"{v:.2e}" for k, v in errs.items()}, "bit-identical:", bool(np.array_equal(qg, qo)), flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-10, (k, v)
+    assert np.array_equal(qg, qo)
+    # the pulse has moved: the comparison is not of two unchanged states
+    assert np.abs(qo[0] - q_init[0]).max() > 1e-6 * np.abs(q_init[0]).max()
+
+
+def test_box24_fp32p_within_bar_200_steps():
+    case = _box(24, 3.0)
+    print(f"[box] 24^3 FP32P vs FP64, 200 steps", flush=True)
+    g64 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP64)
+    g32 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP32P)
+    g64.run(200)
+    g32.run(200)
+    q64, q32 = g64.q(), g32.q()
+    assert g32.info().cfl == pytest.approx(g64.info().cfl, rel=1e-5)
+    g64.close()
+    g32.close()
+    assert np.all(np.isfinite(q32))
+    errs = _pert_errors(case, q32, q64)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-5, (k, v)
+
+
+@pytest.mark.timeout(600)
+def test_box256_full_size_properties():
+    case = _box(256, 8.0)
+    nsteps = 10
+    print(f"[box] 256^3 full size: {case.n_cells} cells, {nsteps} steps, FP64 and FP32P", flush=True)
+    g64 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP64)
+    c0 = g64.conservation()
+    g64.run(nsteps)
+    c1 = g64.conservation()
+    assert g64.n_nonfinite() == 0
+    q64 = g64.q()
+    g64.close()
+    # the pulse (half-width 8 cells) is centred >= 127 cells from every
+    # patch: 1e-3 p_inf exp(-ln2 (127/8)^2) ~ 1e-73 p_inf vanishes against
+    # p_inf, and 10 RK5 steps carry no signal farther than 50 cells, so
+    # every boundary cell and ghost stays exactly freestream and what enters
+    # through the inlets leaves through the outlets
+    print("FP64 integrals drift", (c1 - c0) / np.abs(c0).max(), flush=True)
+    assert np.all(np.abs(c1 - c0) <= 1e-12 * np.abs(c0).max()), (c0, c1)
+    g32 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP32P)
+    g32.run(nsteps)
+    assert g32.n_nonfinite() == 0
+    c32 = g32.conservation()
+    q32 = g32.q()
+    g32.close()
+    print("FP32P - FP64 integrals", (c32 - c1) / np.abs(c1).max(), flush=True)
+    assert np.all(np.abs(c32 - c1) <= 1e-7 * np.abs(c1).max()), (c32, c1)
+    errs = _pert_errors(case, q32, q64)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-5, (k, v)

@@ -98,3 +98,36 @@ def test_solve_driver_post_processing(tmp_path):
     last = os.path.join(d, "%.12g" % r["written"][-1])  # the writer's time name
     assert np.array_equal(_read_scalar(os.path.join(last, "pAvg")), a_ref)
     assert np.allclose(_read_scalar(os.path.join(last, "pRMS")), r_ref, rtol=1e-13, atol=1e-13 * r_ref.max())
+
+
+@pytest.mark.parametrize("name", ["cylinder_M2AUSM_sponge", "box_M1_euler"])
+def test_restart_from_latest_time_continues_the_run(tmp_path, name):
+    """Run N steps with output, restart with startFrom latestTime (Foam::Time
+    as runTimeManagerOF.cpp:12 creates it reads the newest time directory:
+    p, T, U and the sponge distance alpha), run M more: the state equals an
+    uninterrupted N + M step run to the round-off of re-deriving rho, rho U,
+    rho E from the written p, T, U (written with 17 significant digits)."""
+    c = sc.PARITY_CASES[name]()
+    N, M = 4, 3
+    d = str(tmp_path / name)
+    foam.write_case(d, c, end_steps=N + M, write_interval=N)
+    r1 = solve.run_case(d, steps=N, quiet=True)
+    assert r1["steps"] == N and len(r1["written"]) == 1
+    ctl = os.path.join(d, "system", "controlDict")
+    txt = open(ctl).read()
+    assert "startFrom       startTime;" in txt
+    open(ctl, "w").write(txt.replace("startFrom       startTime;", "startFrom       latestTime;"))
+    fc = foam.FoamCase(d)
+    assert fc.start == pytest.approx(N * c.params["dt"], rel=1e-12)
+    if c.alpha is not None:
+        assert np.array_equal(fc.alpha, c.alpha)
+    fc.close()
+    r2 = solve.run_case(d, quiet=True)
+    assert r2["steps"] == M
+    g = capi.GpuSolver.from_case(c)
+    g.run(N + M)
+    q = g.q()
+    g.close()
+    err = np.abs(r2["q"][0] - q).max(axis=1) / np.abs(q).max(axis=1)
+    print(name, "restart vs uninterrupted, max relative per component:", err, flush=True)
+    assert np.all(err <= 1e-12), err

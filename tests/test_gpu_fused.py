@@ -45,6 +45,9 @@ CASES = {
     "cylinder_M2_unset_patch": lambda: _unset_patch(sc.cylinder(solver=1, minmod=0, mu=0.0)),
     "box_M1": lambda: sc.seeded_perturbation(sc.box(6, solver=0)),
     "box_M2_rk4": lambda: sc.seeded_perturbation(sc.box(6, solver=1, rk_order=4)),
+    # adjustTimeStep: the last fused launch forms the next step's deltaT
+    "vortex_M1_adjust": lambda: sc.vortex(16, solver=0, adjust_dt=1, max_co=0.3),
+    "box_M2_adjust_rk3": lambda: sc.seeded_perturbation(sc.box(6, solver=1, rk_order=3, adjust_dt=1, max_co=0.3)),
 }
 
 
@@ -57,6 +60,7 @@ def test_fused_stage_equals_separate_launches(name):
     fused = capi.GpuSolver.from_case(c, precision=capi.PREC_FP32P)
     ref = capi.GpuSolver.from_case(c, precision=capi.PREC_FP32P)
     dt = c.params["dt"]
+    adjust = bool(c.params.get("adjust_dt"))
     for step in range(3):
         fused.run(1)
         ref.prepare_for_timestep()
@@ -66,6 +70,12 @@ def test_fused_stage_equals_separate_launches(name):
             ref.calc_gradients()
             ref.calc_VIS()
             ref.one_rk_step(rk, dt)
+        if adjust:
+            # Mesh::solve: deltaT = compute_dt(maxCo) after the stages
+            # (mesh_solver.cpp:705-717); the fused path formed it in its last
+            # launch and must hold the same value
+            dt = ref.compute_dt(c.params["max_co"])
+            assert fused.info().dt == dt and 0.0 < dt < 100.0, (step, fused.info().dt, dt)
         a, b = fused.q(), ref.q()
         # the unset patch's ghosts (q = 0, rho = 0) make its neighbours
         # non-finite, as in the reference; a ghost the fused stage failed to
@@ -73,8 +83,10 @@ def test_fused_stage_equals_separate_launches(name):
         assert np.all(np.isfinite(a)) != unset, step
         assert np.array_equal(a, b, equal_nan=True), (step, np.nanmax(np.abs(a - b)))
         assert np.array_equal(fused.field(capi.FIELD_DQ), ref.field(capi.FIELD_DQ), equal_nan=True), step
-    # the step's CFL, formed by the last fused launch, is the stand-alone one
+    # the step's CFL (constant deltaT), formed by the last fused launch, is
+    # the stand-alone one
     info = fused.info()
-    assert fused.compute_cfl(info.dt) == info.cfl and (unset or info.cfl > 0)
+    if not adjust:
+        assert fused.compute_cfl(info.dt) == info.cfl and (unset or info.cfl > 0)
     fused.close()
     ref.close()

@@ -92,3 +92,35 @@ def test_budget_forces_queueing_and_all_complete():
     assert res["n_requests"] == 8 and res["all_finite"]
     assert res["peak_resident_bytes"] <= budget + (1 << 22)
     assert res["tts_p99_s"] >= res["tts_p50_s"] > 0
+
+
+def test_config5_stream_at_size():
+    """The serving configuration's own stream (seed 505: 50/50 vortex and
+    cylinder requests, 16 k - 1 M cells log-uniform, Mach 0.1-0.7) at its
+    size range: its first 24 requests, 40 RK5 steps each, under a 2 GiB HBM
+    budget (the requests need ~2.8 GB together, so admission queues).  The largest and
+    smallest member of each scheme class end bit-identical to their lone
+    runs, and the resident device bytes never exceed the budget."""
+    from lfm_public_amd import serving
+    reqs = serving.make_stream(256, seed=505, nsteps=40)[:24]
+    cells = [r.n_cells for r in reqs]
+    assert min(cells) <= 20_000 and max(cells) >= 1_000_000
+    budget = 2 << 30
+    assert sum(serving.est_device_bytes(c) for c in cells) > budget
+    print(f"[serving] config-5 stream: {len(reqs)} requests, {min(cells)}-{max(cells)} cells", flush=True)
+    srv = serving.Server(hbm_budget=budget, chunk_max=16, keep_results=True)
+    res = srv.run(reqs)
+    srv.close()
+    assert res["n_requests"] == len(reqs)
+    assert res["peak_resident_bytes"] <= budget, res["peak_resident_bytes"]
+    assert res["max_batch_members"] >= 2
+    picks = []
+    for kind in ("vortex", "cylinder"):
+        ks = [r for r in reqs if r.kind == kind]
+        picks += [max(ks, key=lambda r: r.n_cells), min(ks, key=lambda r: r.n_cells)]
+    for r in picks:
+        print(f"[serving] alone {r.rid} ({r.kind}, {r.n_cells} cells)", flush=True)
+        qa = _alone(r)
+        qm = srv.results[r.rid]
+        assert np.all(np.isfinite(qa))
+        assert np.array_equal(qa, qm), (r.rid, np.abs(qa - qm).max())
