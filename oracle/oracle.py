@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblfm_oracle.so")
+# LFM_ORACLE_LIB names another build of the oracle (the mutation checks of
+# tests/test_oracle_mutants.py load deliberately broken copies through it)
+LIB_PATH = os.environ.get("LFM_ORACLE_LIB") or os.path.join(HERE, "liblfm_oracle.so")
 
 F_Q, F_DQ, F_DUDX, F_DTDX, F_TAUMC, F_SIGMAU, F_RHOGRAD, F_PGRAD, F_UGRAD, F_RES, F_VOLINV, F_SIGMA, \
     F_CENTRE, F_VOLUME, F_SUBMESH, F_RHOUGRAD, F_RHOEGRAD, F_RPSIGRAD, F_CGRAD = range(19)
@@ -38,6 +40,8 @@ class _Params(C.Structure):
 def build(force: bool = False) -> str:
     """Compile the oracle with gcc (ISO C99, no FMA contraction)."""
     src = os.path.join(HERE, "lfm_oracle.c")
+    if os.environ.get("LFM_ORACLE_LIB"):
+        return LIB_PATH
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
         subprocess.check_call(["make", "-s", "-C", HERE])
     return LIB_PATH

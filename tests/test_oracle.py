@@ -132,6 +132,9 @@ def test_minmod_special_cases(O):
     d = [1.0, 0.0]
     # r = 2 d.grad / (phiN - phiP) - 1 >= 1: linear weight
     assert O.minmod(2, 1.0, 3.0, [2.0, 0.0], d, 0.3, 1.0) == pytest.approx(0.3 * 1.0 + 0.7 * 3.0)
+    # r = 3 > 1: the limiter saturates at 1 (still the linear weight)
+    assert O.minmod(2, 1.0, 3.0, [4.0, 0.0], d, 0.3, 1.0) == pytest.approx(0.3 * 1.0 + 0.7 * 3.0)
+    assert O.minmod(2, 1.0, 3.0, [4.0, 0.0], d, 0.3, -1.0) == pytest.approx(0.3 * 1.0 + 0.7 * 3.0)
     # r = 0 exactly: limiter 0, upwind
     assert O.minmod(2, 1.0, 3.0, [1.0, 0.0], d, 0.3, 1.0) == pytest.approx(1.0)
     # r <= 0 (gradient opposes): pure upwind on the flux side
