@@ -297,7 +297,7 @@ def main():
     ap.add_argument("--warmup-s", type=float, default=2.0, help="minimum warm-up seconds per path")
     ap.add_argument("--workload", default="box3d", choices=sorted(OWN_STEPS))
     ap.add_argument("--solver", type=int, default=2, help="cylinder2d scheme: 2 (M2AUSM + minmod) or 1 (M2)")
-    ap.add_argument("--renumber", default="none", choices=("none", "morton", "rcm"),
+    ap.add_argument("--renumber", default="none", choices=("none", "morton", "rcm", "column"),
                     help="renumber the cells before the run (renumberMesh-style preprocessing)")
     ap.add_argument("--error-steps", type=int, default=None,
                     help="steps after which the FP32 error against FP64 is measured (default: the "

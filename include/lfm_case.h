@@ -27,11 +27,15 @@ extern "C" {
 typedef struct lfm_case lfm_case;
 
 /* Open an OpenFOAM case directory.  rank < 0 reads the undecomposed case
- * (constant/polyMesh, <startTime>/); rank >= 0 reads processor<rank>/ of a
+ * (constant/polyMesh, <start time>/); rank >= 0 reads processor<rank>/ of a
  * decomposed case (its polyMesh with processor patches, faceProcAddressing,
- * optional cellSubmesh, and its <startTime>/ fields), with the dictionaries
- * of the case root.  Replaces CRunTimeManagerOF(rank) + CPolyMeshReaderOF +
- * CInputReader::initialize (info/lfm_solve.cpp:39-114). */
+ * optional cellSubmesh, and its <start time>/ fields), with the dictionaries
+ * of the case root.  The start time follows controlDict startFrom as
+ * Foam::Time reads it: startTime (the entry; the default), firstTime or
+ * latestTime (the smallest / largest numeric time directory of the case or
+ * processor directory, a restart).  Replaces CRunTimeManagerOF(rank) +
+ * CPolyMeshReaderOF + CInputReader::initialize (info/lfm_solve.cpp:39-114,
+ * runTimeManagerOF/runTimeManagerOF.cpp:5-12). */
 int lfm_case_open(const char *case_dir, int32_t rank, lfm_case **out);
 int lfm_case_close(lfm_case *c);
 
@@ -46,7 +50,8 @@ int lfm_case_params(lfm_case *c, lfm_params *prm);
 /* Initial p, T, U (n_cells*3) and alpha (NULL when the case has none) of the
  * start time (cfd_v0.cpp:919-935). */
 int lfm_case_fields(lfm_case *c, const double **p, const double **T, const double **U3, const double **alpha);
-/* controlDict startTime, endTime, deltaT, writeInterval (time steps when
+/* The start time (as startFrom selects it), controlDict endTime, deltaT,
+ * writeInterval (time steps when
  * writeControl is timeStep, else seconds) and writeControl (0 timeStep,
  * 1 runTime/adjustableRunTime). */
 int lfm_case_time(lfm_case *c, double *start, double *end, double *delta_t, double *write_interval,
@@ -60,7 +65,10 @@ int lfm_case_dict_scalar(lfm_case *c, const char *dict, const char *subdict, con
 
 /* Write p, U and T of one time into <case>[/processor<rank>]/<time name>/
  * (ASCII volScalarField / volVectorField, internalField nonuniform, the
- * boundaryField entries of the start-time files, T from p / (rho R)).
+ * boundaryField entries of the start-time files, T from p / (rho R)), with
+ * rho and E, and the sponge distance alpha when the case has one (the
+ * reference reads it AUTO_WRITE, polyMeshReaderOF.cpp:700-709, so a
+ * restart from this time finds it).
  * q is SoA [(D+2)][n_cells] conservatives in original cell order. */
 int lfm_case_write_time(lfm_case *c, double time, const double *q, int32_t dim);
 

@@ -30,7 +30,7 @@ struct KArgs32 {
     const int32_t *fP, *fN;
     const float *S;  // [nf][D]: a face's area vector as one record
     const float *d, *w;
-    const int32_t *g_off, *g_ent, *s_off, *s_ent;
+    const int32_t *g_off, *g_ent;
     const int32_t *ghost_cell, *ghost_role;
     const float *vinv, *sigma;
     const float *smag;  // [nc] Smagorinsky constants, nullptr when laminar
@@ -484,10 +484,14 @@ struct GradGlobal32 {
 // ---------------------------------------------------------------------------
 // face fluxes minus the freestream flux, one thread per face
 // ---------------------------------------------------------------------------
+// the flux of face f from its cells' states and primitives (qP, sp of P;
+// qN, sn of N), whoever loaded them: the face kernel loads both, the
+// one-launch stage holds its own cell's and loads the other
 template <int D, int SOLVER, bool VISC, bool STORED, class GA>
-__device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, const GA &ga, const int gP,
-                                                const int gN, float (&rhs)[D + 2]) {
-    const int P = a.fP[f], N = a.fN[f];
+__device__ __forceinline__ void face_flux32_states(const KArgs32 &a, const int f, const int P, const int N,
+                                                   const float (&qP)[D + 2], const float (&qN)[D + 2],
+                                                   const Prim32<D> &sp, const Prim32<D> &sn, const GA &ga,
+                                                   const int gP, const int gN, float (&rhs)[D + 2]) {
     using R = GradRec32<D, SOLVER == 2>;
     float gr[2][R::kStride];  // gradient records of P and N
     if (VISC || SOLVER == 2) {
@@ -498,13 +502,10 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
     const bool ghost = (unsigned)(N - a.nc) < (unsigned)a.ng_bnd;
     const float w = a.w[f];
     const size_t nt = a.ntot;
-    float S[D], d[D], qP[D + 2], qN[D + 2];
+    float S[D], d[D];
     load_S32<D>(a.S, f, S);
 #pragma unroll
     for (int k = 0; k < D; k++) d[k] = a.d[(size_t)k * a.nf + f];
-    load_q32<D>(a, P, qP);
-    load_q32<D>(a, N, qN);
-    const Prim32<D> sp = prim32<D>(a, qP), sn = prim32<D>(a, qN);
     float Sm2 = 0.f;
 #pragma unroll
     for (int k = 0; k < D; k++) Sm2 += S[k] * S[k];
@@ -753,6 +754,17 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
     }
 }
 
+template <int D, int SOLVER, bool VISC, bool STORED, class GA>
+__device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, const GA &ga, const int gP,
+                                                const int gN, float (&rhs)[D + 2]) {
+    const int P = a.fP[f], N = a.fN[f];
+    float qP[D + 2], qN[D + 2];
+    load_q32<D>(a, P, qP);
+    load_q32<D>(a, N, qN);
+    const Prim32<D> sp = prim32<D>(a, qP), sn = prim32<D>(a, qN);
+    face_flux32_states<D, SOLVER, VISC, STORED>(a, f, P, N, qP, qN, sp, sn, ga, gP, gN, rhs);
+}
+
 template <int D, int SOLVER, bool VISC, bool STORED>
 __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) {
     if (f >= a.nf) return;
@@ -769,8 +781,11 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
 template <int D>
 struct FluxGlobal32 {
     const KArgs32 &a;
-    __device__ __forceinline__ void get(int e, int f, float (&r)[D + 2]) const {
+    __device__ __forceinline__ void begin_cell(int c) const { (void)c; }
+    // the flux of gather entry e (signed entry ent, face f)
+    __device__ __forceinline__ void get(int e, int ent, int f, float (&r)[D + 2]) const {
         (void)e;
+        (void)ent;
         const float4 v = reinterpret_cast<const float4 *>(a.frhs)[f];
         r[0] = v.x;
         r[1] = v.y;
@@ -812,11 +827,12 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
     constexpr int U = GatherUnroll<D>::value;
     int ent[U];
     float fr[U][D + 2];
+    fa.begin_cell(c);
 #pragma unroll
     for (int j = 0; j < U; j++) {
         const int e = e0 + (j < ne ? j : ne - 1);
         ent[j] = a.g_ent[e];
-        fa.get(e, ent[j] >= 0 ? (ent[j] & kFaceMask) : ~ent[j], fr[j]);
+        fa.get(e, ent[j], ent[j] >= 0 ? (ent[j] & kFaceMask) : ~ent[j], fr[j]);
     }
 #pragma unroll
     for (int j = 0; j < U; j++)
@@ -824,7 +840,7 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
     for (int e = e0 + U; e < e0 + ne; e++) {
         const int en = a.g_ent[e];
         float f1[D + 2];
-        fa.get(e, en >= 0 ? (en & kFaceMask) : ~en, f1);
+        fa.get(e, en, en >= 0 ? (en & kFaceMask) : ~en, f1);
         add(en, f1);
     }
     load_q32<D>(a, c, q);
@@ -897,7 +913,7 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, const FA &f
 }
 
 // compute_dt / compute_cfl (cfd_v0.cpp:2887-2942) of one cell in FP64 from
-// its FP32 deviations q: the sum over its faces in slot order, then dt_c
+// its FP32 deviations q: the sum over its faces in gather order, then dt_c
 // (MODE 0) or cfl_c (MODE 1), combined with the block's starting value so a
 // NaN never wins
 template <int D, int MODE>
@@ -922,14 +938,17 @@ __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, cons
         }
         sum += fabs(fl);
     };
-    // slot-ordered sum; the first slots' area vectors fetched together
-    const int e0 = a.s_off[c], ne = a.s_off[c + 1] - e0;
+    // the sum over the cell's faces in gather order (the update's own list,
+    // whose entries and area vectors it has just read; the FP64 path keeps
+    // the reference's slot order); the first faces' area vectors fetched
+    // together
+    const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
     constexpr int U = GatherUnroll<D>::value;
     double Sj[U][D];
 #pragma unroll
     for (int j = 0; j < U; j++) {
-        const int ent = a.s_ent[e0 + (j < ne ? j : ne - 1)];
-        const int f = ent >= 0 ? ent : ~ent;
+        const int ent = a.g_ent[e0 + (j < ne ? j : ne - 1)];
+        const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
         float sf[D];
         load_S32<D>(a.S, f, sf);
 #pragma unroll
@@ -939,8 +958,8 @@ __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, cons
     for (int j = 0; j < U; j++)
         if (j < ne) add(Sj[j]);
     for (int e = e0 + U; e < e0 + ne; e++) {
-        const int ent = a.s_ent[e];
-        const int f = ent >= 0 ? ent : ~ent;
+        const int ent = a.g_ent[e];
+        const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
         float sf[D];
         load_S32<D>(a.S, f, sf);
         double S[D];
@@ -1047,9 +1066,28 @@ __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B
 template <int D, int SOLVER>
 struct FluxInline32 {
     const KArgs32 &a;
-    __device__ __forceinline__ void get(int e, int f, float (&r)[D + 2]) const {
+    // the gathering cell's state and primitives, loaded and formed once per
+    // cell; each face then loads only its other cell
+    mutable float qc[D + 2];
+    mutable Prim32<D> pc;
+    __device__ __forceinline__ void begin_cell(int c) const {
+        load_q32<D>(a, c, qc);
+        pc = prim32<D>(a, qc);
+    }
+    __device__ __forceinline__ void get(int e, int ent, int f, float (&r)[D + 2]) const {
         (void)e;
-        face_flux32_rhs<D, SOLVER, false, false>(a, f, GradGlobal32<D, false>{a}, 0, 0, r);
+        const bool isP = ent >= 0;  // the cell is the face's P (else its N)
+        const int o = isP ? a.fN[f] : a.fP[f];
+        float qo[D + 2], qP[D + 2], qN[D + 2];
+        load_q32<D>(a, o, qo);
+        const Prim32<D> po = prim32<D>(a, qo);
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) {
+            qP[k] = isP ? qc[k] : qo[k];
+            qN[k] = isP ? qo[k] : qc[k];
+        }
+        const Prim32<D> sp = isP ? pc : po, sn = isP ? po : pc;
+        face_flux32_states<D, SOLVER, false, false>(a, f, 0, 0, qP, qN, sp, sn, GradGlobal32<D, false>{a}, 0, 0, r);
     }
 };
 // One RK stage in one launch for inviscid M1 / M2: the face fluxes, the

@@ -102,8 +102,6 @@ int create_fp32p(Solver *s) {
     }
     rc |= up(s, (int32_t **)&a.g_off, L.g_off.data(), nc + 1);
     rc |= up(s, (int32_t **)&a.g_ent, L.g_ent.data(), L.g_ent.size());
-    rc |= up(s, (int32_t **)&a.s_off, L.s_off.data(), nc + 1);
-    rc |= up(s, (int32_t **)&a.s_ent, L.s_ent.data(), L.s_ent.size());
     rc |= up(s, (int32_t **)&a.ghost_cell, L.ghost_cell.data(), L.ng);
     rc |= up(s, (int32_t **)&a.ghost_role, L.ghost_role.data(), L.ng);
     rc |= up(s, (double **)&a.vol_d, L.volume.data(), nc);

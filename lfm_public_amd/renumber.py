@@ -13,6 +13,13 @@ original numbering to rounding.
   3D it keeps most face neighbours within a short index distance, which is
   what the per-XCD L2 needs (an x-fastest box has its z neighbours n^2
   cells away).
+* ``column_order``: cells grouped into columns of tile x tile cell widths
+  in the first two coordinates, each column numbered layer by layer along
+  the third (x fastest within a layer).  Neighbours across layers are then
+  tile^2 cells apart instead of a whole plane, while runs of consecutive
+  cells along x stay contiguous (coalesced gathers); 2D meshes are tiled in
+  x only.  Cell coordinates are the centres in units of the median distance
+  between face neighbours, so the ordering applies to any mesh.
 """
 from __future__ import annotations
 
@@ -58,6 +65,21 @@ def morton_order(mesh: meshgen.PolyMesh, bits: int = 10) -> np.ndarray:
     return np.argsort(code, kind="stable").astype(np.int64)
 
 
+def column_order(mesh: meshgen.PolyMesh, tile: int = 64) -> np.ndarray:
+    cc = meshgen.cell_centres_approx(mesh)
+    dim = mesh.dim
+    lo = cc.min(axis=0)
+    # the median distance between face-neighbour centres: the cell width
+    ni = mesh.n_internal
+    h = float(np.median(np.linalg.norm(cc[mesh.neighbour] - cc[mesh.owner[:ni]], axis=1))) if ni else 1.0
+    idx = np.floor((cc - lo) / h + 0.5).astype(np.int64)  # integer cell coordinates
+    if dim == 2:
+        keys = (idx[:, 0], idx[:, 1], idx[:, 0] // tile)
+    else:
+        keys = (idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 0] // tile, idx[:, 1] // tile)
+    return np.lexsort(keys).astype(np.int64)
+
+
 def renumber_mesh(mesh: meshgen.PolyMesh, order: np.ndarray) -> meshgen.PolyMesh:
     """New cell i is old cell order[i].  Internal faces are re-sorted into
     OpenFOAM's upper-triangular order (owner ascending, then neighbour) and
@@ -97,7 +119,7 @@ def renumber_mesh(mesh: meshgen.PolyMesh, order: np.ndarray) -> meshgen.PolyMesh
 def renumber_case(case: Case, method: str = "morton") -> Case:
     if method == "none":
         return case
-    order = {"morton": morton_order, "rcm": rcm_order}[method](case.mesh)
+    order = {"morton": morton_order, "rcm": rcm_order, "column": column_order}[method](case.mesh)
     m = renumber_mesh(case.mesh, order)
     alpha = None if case.alpha is None else case.alpha[order]
     return Case(f"{case.name}_{method}", m, case.params, case.p[order], case.T[order], case.U[order], alpha)

@@ -52,19 +52,46 @@ def test_fp32p_matches_fp64(name):
         assert i32.cfl == pytest.approx(i64.cfl, rel=1e-5)
 
 
-@pytest.mark.parametrize("name", ["vortex_M1_euler", "cylinder_M2AUSM_sponge", "box_M2AUSM_visc", "box_M1_euler"])
-def test_fp32p_long_run(name):
-    """200 RK5 steps: the error stays under the 1e-5 bar as it accumulates."""
+@pytest.mark.parametrize("name,steps", [("vortex_M1_euler", 200), ("cylinder_M2AUSM_sponge", 200),
+                                        ("box_M2AUSM_visc", 200), ("box_M1_euler", 200),
+                                        ("vortex_M2AUSM_minmod", 100)])
+def test_fp32p_long_run(name, steps):
+    """Long runs: the error stays under the 1e-5 bar as it accumulates.  The
+    24^2 M2AUSM vortex is checked while it is stable (it diverges in FP64
+    between steps 200 and 250, test_vortex24_m2ausm_diverges_in_fp64)."""
     from lfm_public_amd import capi
     case = sc.PARITY_CASES[name]()
     g64 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP64)
     g32 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP32P)
-    g64.run(200)
-    g32.run(200)
+    g64.run(steps)
+    g32.run(steps)
     errs = _errors(case, g32.q(), g64.q())
-    print(name, "200 steps", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    print(name, steps, "steps", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     for k, v in errs.items():
         assert v <= TOL, (k, v)
+
+
+def test_vortex24_m2ausm_diverges_in_fp64():
+    """The 24^2 vortex under M2AUSM + minmod (cells 0.83 wide against a
+    vortex radius of 1, strength 5) is unstable in FP64 itself: the oracle
+    and the FP64 path go non-finite between steps 200 and 250, and by step
+    200 the blow-up has begun (minimum pressure 0.66 p_inf, from 0.88 at step
+    150).  Its 200-step FP32 difference (2.4e-5) is that instability
+    amplifying the first differences, not FP32 error accumulating; the bar
+    is checked on it at 100 steps (test_fp32p_long_run)."""
+    from lfm_public_amd import capi, cases
+    case = sc.PARITY_CASES["vortex_M2AUSM_minmod"]()
+    g, R = cases.gamma_of(case.params), cases.gas_constant(case.params["mol_weight"])
+    g64 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP64)
+    g64.run(150)
+    p150 = primitives(g64.q(), 2, g, R)[0].min()
+    g64.run(50)
+    p200 = primitives(g64.q(), 2, g, R)[0].min()
+    g64.run(50)
+    print("min p / p_inf at 150, 200:", p150 / case.params["p_inf"], p200 / case.params["p_inf"],
+          "non-finite at 250:", g64.n_nonfinite(), flush=True)
+    assert p150 > 0.8 * case.params["p_inf"] and p200 < 0.75 * case.params["p_inf"]
+    assert g64.n_nonfinite() > 0
 
 
 def _fp64_with_fp32_storage(case, nsteps):
@@ -91,16 +118,16 @@ def _fp64_with_fp32_storage(case, nsteps):
     return o.q()
 
 
-def test_fp32p_error_explained_by_fp32_storage(oracle_lib):
-    """The 24^2 vortex under M2AUSM + minmod (cells 0.83 wide against a vortex
-    radius of 1, strength 5) misses the 1e-5 bar after 200 steps, and the
-    reason is the FP32 storage itself: FP64 arithmetic on a state and RK
-    register rounded to FP32 after every stage already departs from FP64 by
-    ~1e-5 (p 8.9e-6, T 1.0e-5, U 1.2e-5), against 3.5e-7 when only the
-    initial state is rounded.  The FP32 path, whose arithmetic rounds as
-    well, must stay within 3x of that storage-only error on every field."""
+def test_fp32p_error_is_the_storage_rounding(oracle_lib):
+    """Where the FP32 path's error comes from, on the smooth 24^2 M1 vortex
+    after 200 steps: FP64 arithmetic on a state and RK register rounded to
+    FP32 after every stage (the FP32 path's storage, nothing else rounded)
+    already departs from FP64 by 2.8e-6 (p), 3.6e-6 (T), 1.8e-6 (U), against
+    1e-7 when only the initial state is rounded.  The FP32 path, which
+    rounds its arithmetic as well, lands within 1.5x of that: its error is
+    the storage precision, not the FP32 arithmetic."""
     from lfm_public_amd import capi
-    case = sc.PARITY_CASES["vortex_M2AUSM_minmod"]()
+    case = sc.PARITY_CASES["vortex_M1_euler"]()
     n = 200
     g64 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP64)
     g32 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP32P)
@@ -111,8 +138,8 @@ def test_fp32p_error_explained_by_fp32_storage(oracle_lib):
     es = _errors(case, _fp64_with_fp32_storage(case, n), q64)
     print("FP32 path / FP32 storage only", {k: f"{e32[k]:.2e}/{es[k]:.2e}" for k in e32}, flush=True)
     for k in e32:
-        assert es[k] > 2e-6, (k, es[k])  # the storage error is what it is claimed to be
-        assert e32[k] <= 3 * es[k], (k, e32[k], es[k])
+        assert es[k] > 1e-6, (k, es[k])
+        assert e32[k] <= 1.5 * es[k], (k, e32[k], es[k])
 
 
 def test_fp32p_freestream_exact():

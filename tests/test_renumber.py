@@ -15,10 +15,10 @@ import oracle as O
 
 
 def _order(method, mesh):
-    return {"morton": renumber.morton_order, "rcm": renumber.rcm_order}[method](mesh)
+    return {"morton": renumber.morton_order, "rcm": renumber.rcm_order, "column": renumber.column_order}[method](mesh)
 
 
-@pytest.mark.parametrize("method", ["morton", "rcm"])
+@pytest.mark.parametrize("method", ["morton", "rcm", "column"])
 @pytest.mark.parametrize("name", ["cylinder_M2AUSM_sponge", "box_M2AUSM_visc", "ogrid_M1_cyclic_rk2",
                                   "vortex_M1_rk3_adjust_res"])
 def test_renumbered_case_matches_original(name, method):
@@ -65,3 +65,23 @@ def test_orders_shorten_neighbour_distance():
         assert spread(r) < base / gain, (method, spread(r), base)
     z = renumber.renumber_mesh(mesh, renumber.morton_order(mesh))
     assert far(z) < far(mesh)
+
+
+def test_column_order_layout():
+    """Column tiles on a 3D box: within a tile, x runs fastest, then y, then
+    the layers along z; neighbours across layers are tile^2 cells apart
+    (instead of a whole n^2 plane), and only faces on tile sides reach
+    farther."""
+    n, t = 16, 4
+    mesh = meshgen.box_3d(n, 1.0)
+    order = renumber.column_order(mesh, tile=t)
+    cc = meshgen.cell_centres_approx(mesh)[order]
+    i, j, k = [np.rint(cc[:, a] + n / 2 - 0.5).astype(int) for a in range(3)]
+    assert np.array_equal(i[:t], np.arange(t)) and np.all(j[:t] == 0) and np.all(k[:t] == 0)
+    assert np.all(k[t * t:2 * t * t] == 1)                          # the next layer of the first column
+    assert np.all((i[: t * t * n] < t) & (j[: t * t * n] < t))     # the whole first column first
+    r = renumber.renumber_mesh(mesh, order)
+    dist = np.abs(r.neighbour.astype(np.int64) - r.owner[: r.n_internal])
+    assert np.count_nonzero(dist == t * t) == n * n * (n - 1)      # every z face
+    side = 2 * n * n * (n // t - 1)                                  # x and y faces on tile sides
+    assert np.count_nonzero(dist > t * t) == side
