@@ -194,6 +194,16 @@ int lfm_get_residual(lfm_solver *s, double *res);
 int lfm_conservation(lfm_solver *s, double *out);
 /* Number of non-finite conservatives in the state (0 = healthy). */
 int lfm_check_finite(lfm_solver *s, int64_t *n_bad);
+/* The same count without allocating anything (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Its per-block partial counts live in a static device array of the
+ * library (65,537 x 8 bytes per device, held until the process ends), so
+ * the handle may have at most 16,777,216 cells (LFM_EINVAL beyond, with a
+ * message naming the limit).  Every call on one device shares that array:
+ * calls that overlap on one device (other host threads, other streams)
+ * race; calls ordered one after another need no clearing between them. */
+int lfm_check_finite_noalloc(lfm_solver *s, int64_t *n_bad);
 
 /* ---------------------------------------------------------------------
  * Halo exchange between partitions (ISolver::mpi_communication / mpi_wait,

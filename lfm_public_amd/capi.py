@@ -27,7 +27,7 @@ EXPORTS = ("lfm_patch_role", "lfm_solver_create", "lfm_solver_destroy", "lfm_las
            "lfm_prepare_for_timestep", "lfm_prepare_for_rkstep", "lfm_set_boundary_conditions",
            "lfm_calc_gradients", "lfm_calc_vis", "lfm_one_rk_step", "lfm_compute_dt", "lfm_compute_cfl", "lfm_run",
            "lfm_synchronize", "lfm_time_steps", "lfm_get_field", "lfm_set_field", "lfm_get_residual",
-           "lfm_check_finite", "lfm_profile_steps", "lfm_conservation", "lfm_batch_create", "lfm_batch_destroy",
+           "lfm_check_finite", "lfm_check_finite_noalloc", "lfm_profile_steps", "lfm_conservation", "lfm_batch_create", "lfm_batch_destroy",
            "lfm_batch_set", "lfm_batch_run", "lfm_batch_synchronize", "lfm_solver_create_part",
            "lfm_halo_neighbours", "lfm_halo_export", "lfm_halo_import", "lfm_halo_sizes", "lfm_halo_pack",
            "lfm_halo_unpack", "lfm_part_group_create", "lfm_part_group_run", "lfm_part_group_synchronize",

@@ -2,6 +2,9 @@
 // reset for a new time step, and the finiteness check.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "kernels_fp32p.hpp"
@@ -174,6 +177,109 @@ int Solver::set_field(int which, const double *in) {
         HIP_TRY(hipMemcpy(r.ptr + k * r.cs, tmp.data(), nc * sizeof(double), hipMemcpyHostToDevice));
     }
     return LFM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The finiteness count without allocating (lfm_check_finite_noalloc): per-
+// block counts in a file-scope device array, one slot per block of 256
+// cells, the total in the slot after them.  Every block writes its own slot
+// and the final kernel writes the total, so nothing needs clearing between
+// calls.  One array per device, shared by every call on it: calls that
+// overlap on one device race.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocBlocks = 65536;  // cells <= 65536 * 256 = 16,777,216
+static __device__ __align__(256) unsigned long long scratch[kNoAllocBlocks + 1];
+
+// the array's device address, looked up once per device with that device
+// current (the host-side name is not a device address)
+static std::mutex g_scratch_mutex;
+static std::vector<unsigned long long *> g_scratch_ptr;
+
+static int scratch_of(int device, unsigned long long **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = (unsigned long long *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return LFM_OK;
+}
+
+// per-block count of non-finite conservatives (cells c < nc of a state in
+// the record layout: components 0-3 at q[4 i + k], the 3D energy at
+// q[4 ntot + i]); every block writes partial[blockIdx.x]
+template <typename T, int D>
+static __global__ void __launch_bounds__(256) nonfinite_blocks(const T *q, int nc, size_t ntot,
+                                                               unsigned long long *partial) {
+    __shared__ unsigned long long red[256 / 64];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long bad = 0;
+    if (c < nc) {
+#pragma unroll
+        for (int k = 0; k < D + 2; k++) bad += !isfinite(q[k < 4 ? 4 * (size_t)c + k : 4 * ntot + c]);
+    }
+    for (int off = 32; off > 0; off >>= 1) bad += __shfl_down(bad, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bad;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// the sum of nb block counts into partial[total_slot] (one block of 1024)
+static __global__ void __launch_bounds__(1024) nonfinite_total(unsigned long long *partial, int nb, int total_slot) {
+    __shared__ unsigned long long red[1024 / 64];
+    unsigned long long v = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) v += partial[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < 1024 / 64; i++) t += red[i];
+        partial[total_slot] = t;
+    }
+}
+
+int Solver::check_finite_noalloc(int64_t *n_bad) {
+    const int nb = (L.nc + 255) / 256;
+    if (nb > kNoAllocBlocks)
+        return fail(LFM_EINVAL, "lfm_check_finite_noalloc: " + std::to_string(L.nc) + " cells exceed its maximum of " +
+                                    std::to_string((long)kNoAllocBlocks * 256) + " (use lfm_check_finite)");
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) HIP_TRY(hipSetDevice(device));
+    unsigned long long *part = nullptr;
+    int rc = scratch_of(device, &part);
+    if (rc == LFM_OK && nb > 0) {
+        const size_t ntot = (size_t)L.nc + L.ng;
+        if (fp32) {
+            if (L.D == 2)
+                hipLaunchKernelGGL((nonfinite_blocks<float, 2>), dim3(nb), dim3(256), 0, stream, k32->q, L.nc, ntot, part);
+            else
+                hipLaunchKernelGGL((nonfinite_blocks<float, 3>), dim3(nb), dim3(256), 0, stream, k32->q, L.nc, ntot, part);
+        } else {
+            if (L.D == 2)
+                hipLaunchKernelGGL((nonfinite_blocks<double, 2>), dim3(nb), dim3(256), 0, stream, k64.q, L.nc, ntot, part);
+            else
+                hipLaunchKernelGGL((nonfinite_blocks<double, 3>), dim3(nb), dim3(256), 0, stream, k64.q, L.nc, ntot, part);
+        }
+        hipLaunchKernelGGL(nonfinite_total, dim3(1), dim3(1024), 0, stream, part, nb, kNoAllocBlocks);
+        hipError_t e = hipGetLastError();
+        // the total into the handle's pinned host scratch
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h_scalar + 40, part + kNoAllocBlocks, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) rc = fail(LFM_EDEVICE, std::string("lfm_check_finite_noalloc: ") + hipGetErrorString(e));
+    }
+    if (rc == LFM_OK) {
+        unsigned long long t = 0;
+        if (nb > 0) std::memcpy(&t, h_scalar + 40, sizeof t);
+        *n_bad = (int64_t)t;
+    }
+    if (cur != device) (void)hipSetDevice(cur);
+    return rc;
 }
 
 int Solver::check_finite(int64_t *n_bad) {

@@ -500,6 +500,7 @@ int lfm_solver_create_part(const lfm_polymesh *mesh, const lfm_partition *part, 
     // the Smagorinsky stress does not scale with mu
     s->visc = prm->mu != 0.0 || !prm->laminar;
     std::lock_guard<std::mutex> lock(lfm::device_setup_mutex());
+    HIP_TRY(hipGetDevice(&s->device));
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&s->ev0));
     HIP_TRY(hipEventCreate(&s->ev1));
@@ -748,6 +749,11 @@ int lfm_conservation(lfm_solver *h, double *out) {
 int lfm_check_finite(lfm_solver *h, int64_t *n_bad) {
     if (!h || !n_bad) return fail(LFM_EINVAL, "null argument");
     return h->impl->check_finite(n_bad);
+}
+
+int lfm_check_finite_noalloc(lfm_solver *h, int64_t *n_bad) {
+    if (!h || !n_bad) return fail(LFM_EINVAL, "null argument");
+    return h->impl->check_finite_noalloc(n_bad);
 }
 
 }  // extern "C"

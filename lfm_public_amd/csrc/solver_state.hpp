@@ -48,6 +48,7 @@ struct Solver {
     bool fp32 = false;
     bool visc = true;  // mu != 0: the step forms the viscous terms
 
+    int device = 0;  // the HIP device current when the handle was created
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double *h_scalar = nullptr;   // pinned host scratch (64 doubles)
@@ -162,6 +163,9 @@ struct Solver {
     int get_field(int which, double *out);
     int set_field(int which, const double *in);
     int check_finite(int64_t *n_bad);
+    // the same count with its per-block partials in the library's static
+    // device scratch instead of an allocated counter
+    int check_finite_noalloc(int64_t *n_bad);
 };
 
 int create_fp32p(Solver *s);
