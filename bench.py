@@ -312,6 +312,8 @@ def main():
                     help="requests of the serving stream run after the timed region (0: none)")
     ap.add_argument("--serve-steps", type=int, default=500)
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--fp32-only", action="store_true",
+                    help="time the FP32 path alone (profiling runs; no FP64 line, no error)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = OWN_STEPS[args.workload]
@@ -335,16 +337,21 @@ def main():
     traffic_key = case.name
 
     # ---- FP64 -------------------------------------------------------------
-    n_err = 0 if args.fp64_only else args.error_steps
-    g64, info64, t64, q64 = time_path(case, capi.PREC_FP64, args.steps, args.warmup, args.warmup_s, nprof, n_err)
-    nf = info64.n_faces
-    ms64 = t64["wall_ms"] / args.steps
-    rate64 = nc * args.steps / (t64["wall_ms"] * 1e-3)
-    log(f"FP64: {ms64:.3f} ms/step ({t64['event_ms'] / args.steps:.3f} by events), {rate64:.3e} cell-updates/s")
-    bad64 = g64.n_nonfinite()
-    g64.close()
-    if bad64:
-        raise SystemExit(f"FP64 state went non-finite ({bad64} values): the workload's dt is unstable")
+    n_err = 0 if (args.fp64_only or args.fp32_only) else args.error_steps
+    have64 = not args.fp32_only
+    q64 = None
+    if have64:
+        g64, info64, t64, q64 = time_path(case, capi.PREC_FP64, args.steps, args.warmup, args.warmup_s, nprof,
+                                          n_err)
+        nf = info64.n_faces
+        ms64 = t64["wall_ms"] / args.steps
+        rate64 = nc * args.steps / (t64["wall_ms"] * 1e-3)
+        log(f"FP64: {ms64:.3f} ms/step ({t64['event_ms'] / args.steps:.3f} by events), {rate64:.3e} "
+            f"cell-updates/s")
+        bad64 = g64.n_nonfinite()
+        g64.close()
+        if bad64:
+            raise SystemExit(f"FP64 state went non-finite ({bad64} values): the workload's dt is unstable")
 
     # ---- FP32 deviation path ---------------------------------------------
     have32 = False
@@ -357,6 +364,7 @@ def main():
         except capi.LfmError as e:
             log(f"FP32-perturbation path unavailable: {e}")
     if have32:
+        nf = info32.n_faces
         ms32 = t32["wall_ms"] / args.steps
         rate32 = nc * args.steps / (t32["wall_ms"] * 1e-3)
         log(f"FP32P: {ms32:.3f} ms/step ({t32['event_ms'] / args.steps:.3f} by events), {rate32:.3e} "
@@ -372,7 +380,7 @@ def main():
         del q32
     del q64
 
-    rl64 = roofline(case, nf, prm64, nstages, t64["prof"], nprof, ms64, 8, traffic_key)
+    rl64 = roofline(case, nf, prm64, nstages, t64["prof"], nprof, ms64, 8, traffic_key) if have64 else None
     if have32:
         value, ms_step, dtype = rate32, ms32, "f32"
         rl = roofline(case, nf, prm32, nstages, t32["prof"], nprof, ms32, 4, traffic_key)
@@ -419,13 +427,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{case.name}: {desc}; {nc} cells, {nf} faces",
                    "cells": nc, "faces": nf, "rk_stages": nstages,
-                   "precision_paths": ["fp64"] + (["fp32p"] if have32 else [])},
-        "fp64": dict(path_rec(t64, rate64, ms64, info64), roofline=rl64),
+                   "precision_paths": (["fp64"] if have64 else []) + (["fp32p"] if have32 else [])},
+        "fp64": dict(path_rec(t64, rate64, ms64, info64), roofline=rl64) if have64 else None,
         "roofline": rl,
         "cpu_baseline": cpu,
     }
     if have32:
-        out["fp32p"] = dict(path_rec(t32, rate32, ms32, info32), speedup_vs_fp64=rate32 / rate64,
+        out["fp32p"] = dict(path_rec(t32, rate32, ms32, info32), speedup_vs_fp64=rate32 / rate64 if have64 else None,
                             error_vs_fp64=err, error_steps=n_err if err else None)
     if args.secondary != "none" and args.secondary != args.workload and not args.fp64_only:
         out["secondary"] = secondary_run(args.secondary, args.solver)

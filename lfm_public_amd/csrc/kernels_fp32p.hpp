@@ -50,6 +50,12 @@ struct KArgs32 {
     const double *vol_d;  // FP64 cell volumes (conservation integrals)
     double *d_cfl, *d_dtmin;  // this solver's FP64 CFL / dt-min scalars
     int nblk_cells;           // blocks of 256 over the cells (reduction partials)
+    // per-cell face slots of the one-launch stage (the first GatherUnroll
+    // gather entries of each cell, slot j of cell c at [j][c]): the face's
+    // other cell (~other where the cell is the face's N), its area vector
+    // ([j][k][c], face orientation) and weight; nullptr until built
+    const int32_t *sl_nb;
+    const float *sl_S, *sl_w;
     float gamma, gm1, gog, R_inv, mu, cp, pr_inv;
     float rho_inf, p_inf, T_inf, Rpsi_inf, E_inf, H_inf, G_inf, c_inf;
     float uinf[3], minf[3];
@@ -301,6 +307,17 @@ inline int grad_record_stride(int D, bool rg) {
     return (n + 1) / 2 * 2;
 }
 
+// Gather entries whose loads the FP32 gradient and update kernels issue
+// together before any arithmetic (the rest follow one at a time).
+template <int D>
+struct GradUnroll32 {
+    static constexpr int value = GatherUnroll<D>::value;
+};
+template <int D>
+struct UpdateUnroll32 {
+    static constexpr int value = GatherUnroll<D>::value;
+};
+
 // Green-Gauss sums of one cell over its faces in gather order; returns the
 // cell's own primitive state
 template <int D, int GRAD, bool VISC>
@@ -348,9 +365,9 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
         }
     };
     const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
-    // the first GatherUnroll faces: every load issued before the sums (a
-    // short list repeats its last face, whose terms are then not added)
-    constexpr int U = GatherUnroll<D>::value;
+    // the first faces: every load issued before the sums (a short list
+    // repeats its last face, whose terms are then not added)
+    constexpr int U = GradUnroll32<D>::value;
     bool isP[U];
     float w[U], S[U][D], qo[U][D + 2];
 #pragma unroll
@@ -490,7 +507,8 @@ struct GradGlobal32 {
 template <int D, int SOLVER, bool VISC, bool STORED, class GA>
 __device__ __forceinline__ void face_flux32_states(const KArgs32 &a, const int f, const int P, const int N,
                                                    const float (&qP)[D + 2], const float (&qN)[D + 2],
-                                                   const Prim32<D> &sp, const Prim32<D> &sn, const GA &ga,
+                                                   const Prim32<D> &sp, const Prim32<D> &sn,
+                                                   const float (&S)[D], const float w, const GA &ga,
                                                    const int gP, const int gN, float (&rhs)[D + 2]) {
     using R = GradRec32<D, SOLVER == 2>;
     float gr[2][R::kStride];  // gradient records of P and N
@@ -500,10 +518,8 @@ __device__ __forceinline__ void face_flux32_states(const KArgs32 &a, const int f
     }
     // is_ghost: boundary ghosts only; halo ghosts carry the neighbour's tensors
     const bool ghost = (unsigned)(N - a.nc) < (unsigned)a.ng_bnd;
-    const float w = a.w[f];
     const size_t nt = a.ntot;
-    float S[D], d[D];
-    load_S32<D>(a.S, f, S);
+    float d[D];
 #pragma unroll
     for (int k = 0; k < D; k++) d[k] = a.d[(size_t)k * a.nf + f];
     float Sm2 = 0.f;
@@ -762,7 +778,9 @@ __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, c
     load_q32<D>(a, P, qP);
     load_q32<D>(a, N, qN);
     const Prim32<D> sp = prim32<D>(a, qP), sn = prim32<D>(a, qN);
-    face_flux32_states<D, SOLVER, VISC, STORED>(a, f, P, N, qP, qN, sp, sn, ga, gP, gN, rhs);
+    float S[D];
+    load_S32<D>(a.S, f, S);
+    face_flux32_states<D, SOLVER, VISC, STORED>(a, f, P, N, qP, qN, sp, sn, S, a.w[f], ga, gP, gN, rhs);
 }
 
 template <int D, int SOLVER, bool VISC, bool STORED>
@@ -778,14 +796,51 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
 // ordered gather of face fluxes, sponge, RK update (deviations)
 // ---------------------------------------------------------------------------
 // Where the update reads the face fluxes (the solver's frhs array, by face)
+// What the ordered gather of a cell (cell_update32_cell) asks of its source
+// of face data, for gather entry e (the cell's j-th, j < kUnroll and known
+// at compile time), and for the entries after those (the *_far forms):
+//   key(e, j, c)       an entry key, >= 0 where the cell is the face's P;
+//   ghost(key)         the boundary ghost behind a P entry, else -1;
+//   get(e, j, key, r)  the face's flux;
+//   area(e, j, c, S)   the face's area vector, negated on N entries.
+// FaceList32 reads the gather list (key = the signed face index with the
+// ghost flag); FluxGlobal32 adds the stored face fluxes.
 template <int D>
-struct FluxGlobal32 {
+struct FaceList32 {
     const KArgs32 &a;
+    // gather entries whose loads are issued together before any arithmetic
+    static constexpr int kUnroll = UpdateUnroll32<D>::value;
     __device__ __forceinline__ void begin_cell(int c) const { (void)c; }
-    // the flux of gather entry e (signed entry ent, face f)
-    __device__ __forceinline__ void get(int e, int ent, int f, float (&r)[D + 2]) const {
+    __device__ __forceinline__ int key(int e, int j, int c) const {
+        (void)j;
+        (void)c;
+        return a.g_ent[e];
+    }
+    __device__ __forceinline__ int ghost(int key) const {
+        return (key >= 0 && (key & kGhostFaceFlag)) ? a.fN[key & kFaceMask] - a.nc : -1;
+    }
+    __device__ __forceinline__ static int face(int key) { return key >= 0 ? (key & kFaceMask) : ~key; }
+    __device__ __forceinline__ void area(int e, int j, int c, double (&S)[D]) const {
+        const int key = this->key(e, j, c);
+        float sf[D];
+        load_S32<D>(a.S, face(key), sf);
+#pragma unroll
+        for (int k = 0; k < D; k++) S[k] = key >= 0 ? (double)sf[k] : -(double)sf[k];
+    }
+    __device__ __forceinline__ int key_far(int e, int j, int c) const { return key(e, j, c); }
+    __device__ __forceinline__ void area_far(int e, int j, int c, double (&S)[D]) const { area(e, j, c, S); }
+};
+
+template <int D>
+struct FluxGlobal32 : FaceList32<D> {
+    using FaceList32<D>::a;
+    using FaceList32<D>::face;
+    __device__ __forceinline__ void get_far(int e, int j, int key, float (&r)[D + 2]) const { get(e, j, key, r); }
+    // the flux of gather entry e
+    __device__ __forceinline__ void get(int e, int j, int key, float (&r)[D + 2]) const {
         (void)e;
-        (void)ent;
+        (void)j;
+        const int f = face(key);
         const float4 v = reinterpret_cast<const float4 *>(a.frhs)[f];
         r[0] = v.x;
         r[1] = v.y;
@@ -811,11 +866,11 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
         res[k] = 0.f;
     }
     const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
-    int gface[kMaxFaces], ngf = 0;
-    auto add = [&](const int ent, const float (&fr)[D + 2]) {
-        const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
-        if (ent >= 0 && (ent & kGhostFaceFlag) && ngf < kMaxFaces) gface[ngf++] = f;
-        const float sg = ent >= 0 ? 1.f : -1.f;
+    int gidx[kMaxFaces], ngf = 0;  // the cell's boundary ghosts
+    auto add = [&](const int key, const float (&fr)[D + 2]) {
+        const int g = fa.ghost(key);
+        if (g >= 0 && ngf < kMaxFaces) gidx[ngf++] = g;
+        const float sg = key >= 0 ? 1.f : -1.f;
 #pragma unroll
         for (int k = 0; k < D + 2; k++) {
             const float r = sg * fr[k];
@@ -824,24 +879,24 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
         }
     };
     // the first faces' fluxes fetched together, then added in gather order
-    constexpr int U = GatherUnroll<D>::value;
-    int ent[U];
+    constexpr int U = FA::kUnroll;
+    int key[U];
     float fr[U][D + 2];
     fa.begin_cell(c);
 #pragma unroll
     for (int j = 0; j < U; j++) {
-        const int e = e0 + (j < ne ? j : ne - 1);
-        ent[j] = a.g_ent[e];
-        fa.get(e, ent[j], ent[j] >= 0 ? (ent[j] & kFaceMask) : ~ent[j], fr[j]);
+        const int jj = j < ne ? j : ne - 1;
+        key[j] = fa.key(e0 + jj, jj, c);
+        fa.get(e0 + jj, jj, key[j], fr[j]);
     }
 #pragma unroll
     for (int j = 0; j < U; j++)
-        if (j < ne) add(ent[j], fr[j]);
+        if (j < ne) add(key[j], fr[j]);
     for (int e = e0 + U; e < e0 + ne; e++) {
-        const int en = a.g_ent[e];
+        const int kj = fa.key_far(e, e - e0, c);
         float f1[D + 2];
-        fa.get(e, en, en >= 0 ? (en & kFaceMask) : ~en, f1);
-        add(en, f1);
+        fa.get_far(e, e - e0, kj, f1);
+        add(kj, f1);
     }
     load_q32<D>(a, c, q);
     const float sgm = a.sigma[c];
@@ -854,7 +909,7 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
     store_q32<D>(qout, a.ntot, c, q);
     // refresh this cell's boundary ghosts for the next stage
     for (int i = 0; i < ngf; i++) {
-        const int g = a.fN[gface[i]] - a.nc;
+        const int g = gidx[i];
         float qg[D + 2];
         if (!ghost_state32<D>(a, a.ghost_role[g], q, qg)) {
             if (qout == a.q) continue;
@@ -864,8 +919,9 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
     }
 }
 
-template <int D, int MODE>
-__device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, const float (&qc)[D + 2], double param);
+template <int D, int MODE, class FA>
+__device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const FA &fa, const int c, const float (&qc)[D + 2],
+                                             double param);
 
 // every thread of the block must call this (block-wide reductions when RES
 // or CFL).  CFL >= 0 (the last stage of a step): the step's compute_dt (0) /
@@ -882,7 +938,7 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, const FA &f
     if (c < a.nc) {
         float res[D + 2], q[D + 2];
         cell_update32_cell<D, RES>(a, fa, A, B, zero_dq, c, qout, res, q);
-        if constexpr (CFL >= 0) cv = cfl_cell32<D, CFL>(a, c, q, cfl_param);
+        if constexpr (CFL >= 0) cv = cfl_cell32<D, CFL>(a, fa, c, q, cfl_param);
         if (RES) {
 #pragma unroll
             for (int k = 0; k < D + 2; k++) {
@@ -916,8 +972,9 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, const FA &f
 // its FP32 deviations q: the sum over its faces in gather order, then dt_c
 // (MODE 0) or cfl_c (MODE 1), combined with the block's starting value so a
 // NaN never wins
-template <int D, int MODE>
-__device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, const float (&qc)[D + 2], double param) {
+template <int D, int MODE, class FA>
+__device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const FA &fa, const int c, const float (&qc)[D + 2],
+                                             double param) {
     const double rho = a.rho_inf_d + (double)qc[0];
     double m[D];
 #pragma unroll
@@ -947,24 +1004,15 @@ __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const int c, cons
     double Sj[U][D];
 #pragma unroll
     for (int j = 0; j < U; j++) {
-        const int ent = a.g_ent[e0 + (j < ne ? j : ne - 1)];
-        const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
-        float sf[D];
-        load_S32<D>(a.S, f, sf);
-#pragma unroll
-        for (int k = 0; k < D; k++) Sj[j][k] = ent >= 0 ? (double)sf[k] : -(double)sf[k];
+        const int jj = j < ne ? j : ne - 1;
+        fa.area(e0 + jj, jj, c, Sj[j]);
     }
 #pragma unroll
     for (int j = 0; j < U; j++)
         if (j < ne) add(Sj[j]);
     for (int e = e0 + U; e < e0 + ne; e++) {
-        const int ent = a.g_ent[e];
-        const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
-        float sf[D];
-        load_S32<D>(a.S, f, sf);
         double S[D];
-#pragma unroll
-        for (int k = 0; k < D; k++) S[k] = ent >= 0 ? (double)sf[k] : -(double)sf[k];
+        fa.area_far(e, e - e0, c, S);
         add(S);
     }
     const double vol = a.vol_d[c];
@@ -987,7 +1035,7 @@ __device__ __forceinline__ void cfl_dt_cells32_body(const KArgs32 &a, double par
     if (c < a.nc) {
         float q[D + 2];
         load_q32<D>(a, c, q);
-        v = cfl_cell32<D, MODE>(a, c, q, param);
+        v = cfl_cell32<D, MODE>(a, FaceList32<D>{a}, c, q, param);
     }
     block_extremum<MODE>(v, a.partial + lblk);
 }
@@ -1061,23 +1109,48 @@ __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B
 }
 // Where a fused stage takes a face's flux: computed where it is gathered, by
 // the face kernel's own function, from the stage's input state a.q
-// (inviscid M1 / M2: no gradients).  Each interior face is computed by both
-// of its cells, to the same bits.
+// (inviscid M1 / M2: no gradients), with the gathering cell's state and
+// primitives loaded and formed once per cell.  The face data come from the
+// cell's slots (a.sl_*: streamed, one record per gather entry) for its first
+// GatherUnroll entries and from the face arrays beyond them.  Each interior
+// face is computed by both of its cells, to the same bits as the face
+// kernel's (the same function of the same values).
 template <int D, int SOLVER>
-struct FluxInline32 {
+struct FluxSlots32 {
     const KArgs32 &a;
-    // the gathering cell's state and primitives, loaded and formed once per
-    // cell; each face then loads only its other cell
+    // LDS for the area vectors the flux gather reads, kept for the CFL/dt
+    // sums after the update ([U][D][256] per block; nullptr: read again)
+    float *keep_S;
+    static constexpr int U = GatherUnroll<D>::value;  // the slots per cell
+    // 3D: two faces' loads issued together, which leaves 71 VGPRs and 7
+    // waves per SIMD (256^3 box: 0.82 -> 0.75 ms per stage against all six
+    // at 4 waves)
+    static constexpr int kUnroll = D == 3 ? 2 : U;
     mutable float qc[D + 2];
     mutable Prim32<D> pc;
+    mutable int c_ = 0;
     __device__ __forceinline__ void begin_cell(int c) const {
+        c_ = c;
         load_q32<D>(a, c, qc);
         pc = prim32<D>(a, qc);
     }
-    __device__ __forceinline__ void get(int e, int ent, int f, float (&r)[D + 2]) const {
+    // key: the face's other cell, ~other where the cell is the face's N
+    __device__ __forceinline__ int key(int e, int j, int c) const {
         (void)e;
-        const bool isP = ent >= 0;  // the cell is the face's P (else its N)
-        const int o = isP ? a.fN[f] : a.fP[f];
+        return a.sl_nb[(size_t)j * a.nc + c];
+    }
+    // entries after the unrolled ones (j known at run time): the slots up
+    // to U, the face arrays beyond
+    __device__ __forceinline__ int key_far(int e, int j, int c) const {
+        if (j < U) return a.sl_nb[(size_t)j * a.nc + c];
+        const int ent = a.g_ent[e];
+        return ent >= 0 ? a.fN[ent & kFaceMask] : ~a.fP[~ent];
+    }
+    // boundary ghosts sit after the cells (no halo ghosts on this path)
+    __device__ __forceinline__ int ghost(int key) const { return key >= a.nc ? key - a.nc : -1; }
+    __device__ __forceinline__ void flux(int key, const float (&S)[D], float w, float (&r)[D + 2]) const {
+        const bool isP = key >= 0;
+        const int o = isP ? key : ~key;
         float qo[D + 2], qP[D + 2], qN[D + 2];
         load_q32<D>(a, o, qo);
         const Prim32<D> po = prim32<D>(a, qo);
@@ -1087,19 +1160,96 @@ struct FluxInline32 {
             qN[k] = isP ? qo[k] : qc[k];
         }
         const Prim32<D> sp = isP ? pc : po, sn = isP ? po : pc;
-        face_flux32_states<D, SOLVER, false, false>(a, f, 0, 0, qP, qN, sp, sn, GradGlobal32<D, false>{a}, 0, 0, r);
+        face_flux32_states<D, SOLVER, false, false>(a, 0, 0, 0, qP, qN, sp, sn, S, w, GradGlobal32<D, false>{a}, 0,
+                                                     0, r);
+    }
+    // slot j (< U, a compile-time index where called)
+    __device__ __forceinline__ void get(int e, int j, int key, float (&r)[D + 2]) const {
+        (void)e;
+        float S[D];
+#pragma unroll
+        for (int k = 0; k < D; k++) S[k] = a.sl_S[((size_t)j * D + k) * a.nc + c_];
+        if (keep_S) {
+#pragma unroll
+            for (int k = 0; k < D; k++) keep_S[(j * D + k) * 256 + threadIdx.x] = S[k];
+        }
+        flux(key, S, a.sl_w[(size_t)j * a.nc + c_], r);
+    }
+    __device__ __forceinline__ void get_far(int e, int j, int key, float (&r)[D + 2]) const {
+        float S[D], w;
+        far_data(e, j, S, w);
+        flux(key, S, w, r);
+    }
+    __device__ __forceinline__ void far_data(int e, int j, float (&S)[D], float &w) const {
+        if (j < U) {
+#pragma unroll
+            for (int k = 0; k < D; k++) S[k] = a.sl_S[((size_t)j * D + k) * a.nc + c_];
+            w = a.sl_w[(size_t)j * a.nc + c_];
+            if (keep_S) {
+#pragma unroll
+                for (int k = 0; k < D; k++) keep_S[(j * D + k) * 256 + threadIdx.x] = S[k];
+            }
+            return;
+        }
+        const int ent = a.g_ent[e];
+        const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
+        load_S32<D>(a.S, f, S);
+        w = a.w[f];
+    }
+    __device__ __forceinline__ void area(int e, int j, int c, double (&Sd)[D]) const {
+        const bool isP = key(e, j, c) >= 0;
+        float S[D];
+#pragma unroll
+        for (int k = 0; k < D; k++)
+            S[k] = keep_S ? keep_S[(j * D + k) * 256 + threadIdx.x] : a.sl_S[((size_t)j * D + k) * a.nc + c];
+#pragma unroll
+        for (int k = 0; k < D; k++) Sd[k] = isP ? (double)S[k] : -(double)S[k];
+    }
+    __device__ __forceinline__ void area_far(int e, int j, int c, double (&Sd)[D]) const {
+        const bool isP = key_far(e, j, c) >= 0;
+        float S[D], w;
+        if (keep_S && j < U) {
+#pragma unroll
+            for (int k = 0; k < D; k++) S[k] = keep_S[(j * D + k) * 256 + threadIdx.x];
+        } else {
+            far_data(e, j, S, w);
+        }
+#pragma unroll
+        for (int k = 0; k < D; k++) Sd[k] = isP ? (double)S[k] : -(double)S[k];
     }
 };
+
+// the one-launch stage's face slots from the gather list: slot j of cell c
+// holds its j-th gather entry's other cell (~other on N entries), area
+// vector and weight
+template <int D>
+__global__ void __launch_bounds__(256) build_slots32(KArgs32 a, int32_t *nb, float *S, float *w) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nc) return;
+    const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
+    constexpr int U = GatherUnroll<D>::value;
+    for (int j = 0; j < U && j < ne; j++) {
+        const int ent = a.g_ent[e0 + j];
+        const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
+        nb[(size_t)j * a.nc + c] = ent >= 0 ? a.fN[f] : ~a.fP[f];
+#pragma unroll
+        for (int k = 0; k < D; k++) S[((size_t)j * D + k) * a.nc + c] = a.S[(size_t)f * D + k];
+        w[(size_t)j * a.nc + c] = a.w[f];
+    }
+}
+
 // One RK stage in one launch for inviscid M1 / M2: the face fluxes, the
 // ordered gather, sponge and update of cell_update32, reading the state in
-// a.q and writing the new state and boundary ghosts to qout (another buffer)
-// 3D: 4 waves per SIMD (127 instead of 130 VGPRs, no spills; 256^3 box
-// 5.25 -> 4.93 ms per step)
+// a.q and writing the new state and boundary ghosts to qout (another buffer).
+// 3D: 7 waves per SIMD (6 for the last stage, whose CFL/dt sums keep more
+// live), with two faces' loads in flight per wave (kUnroll).
 template <int D, int SOLVER, int CFL>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 3 ? 4 : 1))) stage_fused32(KArgs32 a, float *qout, float A, float B, int zero_dq,
-                                                     double cfl_param) {
-    cell_update32_body<D, false, CFL>(a, FluxInline32<D, SOLVER>{a}, qout, A, B, zero_dq, LFM_TID(blockIdx.x),
-                                      LFM_BLK(), cfl_param);
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 3 ? (CFL < 0 ? 7 : 6) : 1)))
+stage_fused32(KArgs32 a, float *qout, float A, float B, int zero_dq, double cfl_param) {
+    // the last stage keeps its cells' area vectors in LDS for the CFL/dt sums
+    __shared__ float keep[CFL >= 0 ? GatherUnroll<D>::value * D * 256 : 1];
+    cell_update32_body<D, false, CFL>(a, FluxSlots32<D, SOLVER>{a, CFL >= 0 ? keep : nullptr}, qout, A, B, zero_dq,
+                                      LFM_TID(blockIdx.x), LFM_BLK(), cfl_param);
 }
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
@@ -1201,6 +1351,8 @@ void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool
                      const Solver::CflJob *cfl);
 template <int D>
 void launch_cfl_dt32(Solver *s, int mode, double param, double *out, double *dt_dst);
+template <int D>
+void launch_build_slots32(Solver *s, int32_t *nb, float *S, float *w);
 template <int D>
 void launch_stage_fused32(Solver *s, double A, double B, bool zero_dq, const float *qin, float *qout,
                           const Solver::CflJob *cfl);

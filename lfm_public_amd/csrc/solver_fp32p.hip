@@ -257,6 +257,11 @@ void launch_stage_fused32(Solver *s, double A, double B, bool zero_dq, const flo
 }
 
 template <int D>
+void launch_build_slots32(Solver *s, int32_t *nb, float *S, float *w) {
+    hipLaunchKernelGGL(build_slots32<D>, dim3(nblocks(s->L.nc)), dim3(kBlock), 0, s->stream, *s->k32, nb, S, w);
+}
+
+template <int D>
 void launch_cfl_dt32(Solver *s, int mode, double param, double *out, double *dt_dst) {
     const int nb = nblocks(s->L.nc);
     if (mode == 0) {
@@ -390,6 +395,8 @@ template void launch_stage_fused32<2>(Solver *, double, double, bool, const floa
                                      const Solver::CflJob *);
 template void launch_stage_fused32<3>(Solver *, double, double, bool, const float *, float *,
                                      const Solver::CflJob *);
+template void launch_build_slots32<2>(Solver *, int32_t *, float *, float *);
+template void launch_build_slots32<3>(Solver *, int32_t *, float *, float *);
 template void launch_cfl_dt32<2>(Solver *, int, double, double *, double *);
 template void launch_cfl_dt32<3>(Solver *, int, double, double *, double *);
 
