@@ -50,8 +50,9 @@ struct KArgs32 {
     const double *vol_d;  // FP64 cell volumes (conservation integrals)
     double *d_cfl, *d_dtmin;  // this solver's FP64 CFL / dt-min scalars
     int nblk_cells;           // blocks of 256 over the cells (reduction partials)
-    // per-cell face slots of the one-launch stage (the first GatherUnroll
-    // gather entries of each cell, slot j of cell c at [j][c]): the face's
+    // per-cell face slots of the one-launch stage and the gradient kernel
+    // (the first GatherUnroll gather entries of each cell, slot j of cell c
+    // at [j][c]): the face's
     // other cell (~other where the cell is the face's N), its area vector
     // ([j][k][c], face orientation) and weight; nullptr until built
     const int32_t *sl_nb;
@@ -318,9 +319,33 @@ struct UpdateUnroll32 {
     static constexpr int value = GatherUnroll<D>::value;
 };
 
+// gather entry j of cell c (entry e of the list): its side, other cell,
+// weight and area vector.  SL: from the per-cell face slots for the first
+// GatherUnroll entries (coalesced [j][cell] arrays, no face index to chase);
+// otherwise, and beyond them, from the gather list and the face arrays.
+template <int D, bool SL>
+__device__ __forceinline__ void gather_entry32(const KArgs32 &a, const int c, const int e, const int j, bool &isP,
+                                               int &other, float &w, float (&S)[D]) {
+    if (SL && j < GatherUnroll<D>::value) {
+        const int key = a.sl_nb[(size_t)j * a.nc + c];
+        isP = key >= 0;
+        other = isP ? key : ~key;
+        w = a.sl_w[(size_t)j * a.nc + c];
+#pragma unroll
+        for (int k = 0; k < D; k++) S[k] = a.sl_S[((size_t)j * D + k) * a.nc + c];
+        return;
+    }
+    const int ent = a.g_ent[e];
+    isP = ent >= 0;
+    const int f = isP ? (ent & kFaceMask) : ~ent;
+    other = isP ? a.fN[f] : a.fP[f];
+    w = a.w[f];
+    load_S32<D>(a.S, f, S);
+}
+
 // Green-Gauss sums of one cell over its faces in gather order; returns the
 // cell's own primitive state
-template <int D, int GRAD, bool VISC>
+template <int D, int GRAD, bool VISC, bool SL>
 __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, float (&dudx)[D][D],
                                                   float (&dTdx)[D], float (&rg)[D], float (&pg)[D],
                                                   float (&ug)[D][D]) {
@@ -372,32 +397,30 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
     float w[U], S[U][D], qo[U][D + 2];
 #pragma unroll
     for (int j = 0; j < U; j++) {
-        const int ent = a.g_ent[e0 + (j < ne ? j : ne - 1)];
-        isP[j] = ent >= 0;
-        const int f = isP[j] ? (ent & kFaceMask) : ~ent;
-        const int other = isP[j] ? a.fN[f] : a.fP[f];
-        w[j] = a.w[f];
-        load_S32<D>(a.S, f, S[j]);
+        // past the cell's last entry: the last entry again (list) or the
+        // slot's placeholder (slots), not added
+        const int jj = j < ne ? j : ne - 1;
+        int other;
+        gather_entry32<D, SL>(a, c, e0 + jj, SL ? j : jj, isP[j], other, w[j], S[j]);
         load_q32<D>(a, other, qo[j]);
     }
 #pragma unroll
     for (int j = 0; j < U; j++)
         if (j < ne) add(isP[j], w[j], S[j], qo[j]);
     for (int e = e0 + U; e < e0 + ne; e++) {
-        const int ent = a.g_ent[e];
-        const bool p = ent >= 0;
-        const int f = p ? (ent & kFaceMask) : ~ent;
-        float Sf[D], q[D + 2];
-        load_S32<D>(a.S, f, Sf);
-        load_q32<D>(a, p ? a.fN[f] : a.fP[f], q);
-        add(p, a.w[f], Sf, q);
+        bool p;
+        int other;
+        float wf, Sf[D], q[D + 2];
+        gather_entry32<D, SL>(a, c, e, e - e0, p, other, wf, Sf);
+        load_q32<D>(a, other, q);
+        add(p, wf, Sf, q);
     }
     return own;
 }
 
 // every thread of the block must call this (the block's records are written
 // through LDS); c = the block's first cell + threadIdx.x
-template <int D, int GRAD, bool VISC, bool RG>
+template <int D, int GRAD, bool VISC, bool RG, bool SL = false>
 __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
     using R = GradRec32<D, RG>;
     const bool valid = c < a.nc;
@@ -406,7 +429,7 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
     for (int k = 0; k < R::kStride; k++) r[k] = 0.f;
     if (valid) {
         float dudx[D][D], dTdx[D], rg[D], pg[D], ug[D][D];
-        const Prim32<D> own = grad32_sums<D, GRAD, VISC>(a, c, dudx, dTdx, rg, pg, ug);
+        const Prim32<D> own = grad32_sums<D, GRAD, VISC, SL>(a, c, dudx, dTdx, rg, pg, ug);
         // tauMC / sigmaU stored only where a halo payload needs them (otherwise
         // each face recomputes them from dudx); U_grad equals dudx bit for bit
         // (same face velocity, same sums) and shares its storage unless a halo
@@ -795,20 +818,19 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
 // ---------------------------------------------------------------------------
 // ordered gather of face fluxes, sponge, RK update (deviations)
 // ---------------------------------------------------------------------------
-// Where the update reads the face fluxes (the solver's frhs array, by face)
 // What the ordered gather of a cell (cell_update32_cell) asks of its source
-// of face data, for gather entry e (the cell's j-th, j < kUnroll and known
-// at compile time), and for the entries after those (the *_far forms):
+// of face data, for the cell's j-th gather entry e (j < kMaxFaces, known at
+// compile time wherever the gather calls these):
 //   key(e, j, c)       an entry key, >= 0 where the cell is the face's P;
 //   ghost(key)         the boundary ghost behind a P entry, else -1;
 //   get(e, j, key, r)  the face's flux;
 //   area(e, j, c, S)   the face's area vector, negated on N entries.
-// FaceList32 reads the gather list (key = the signed face index with the
-// ghost flag); FluxGlobal32 adds the stored face fluxes.
+// kUnroll entries have their loads issued together.  FaceList32 reads the
+// gather list (key = the signed face index with the ghost flag);
+// FluxGlobal32 adds the stored face fluxes.
 template <int D>
 struct FaceList32 {
     const KArgs32 &a;
-    // gather entries whose loads are issued together before any arithmetic
     static constexpr int kUnroll = UpdateUnroll32<D>::value;
     __device__ __forceinline__ void begin_cell(int c) const { (void)c; }
     __device__ __forceinline__ int key(int e, int j, int c) const {
@@ -820,22 +842,18 @@ struct FaceList32 {
         return (key >= 0 && (key & kGhostFaceFlag)) ? a.fN[key & kFaceMask] - a.nc : -1;
     }
     __device__ __forceinline__ static int face(int key) { return key >= 0 ? (key & kFaceMask) : ~key; }
-    __device__ __forceinline__ void area(int e, int j, int c, double (&S)[D]) const {
+    __device__ __forceinline__ void area(int e, int j, int c, float (&S)[D]) const {
         const int key = this->key(e, j, c);
-        float sf[D];
-        load_S32<D>(a.S, face(key), sf);
+        load_S32<D>(a.S, face(key), S);
 #pragma unroll
-        for (int k = 0; k < D; k++) S[k] = key >= 0 ? (double)sf[k] : -(double)sf[k];
+        for (int k = 0; k < D; k++) S[k] = key >= 0 ? S[k] : -S[k];
     }
-    __device__ __forceinline__ int key_far(int e, int j, int c) const { return key(e, j, c); }
-    __device__ __forceinline__ void area_far(int e, int j, int c, double (&S)[D]) const { area(e, j, c, S); }
 };
 
 template <int D>
 struct FluxGlobal32 : FaceList32<D> {
     using FaceList32<D>::a;
     using FaceList32<D>::face;
-    __device__ __forceinline__ void get_far(int e, int j, int key, float (&r)[D + 2]) const { get(e, j, key, r); }
     // the flux of gather entry e
     __device__ __forceinline__ void get(int e, int j, int key, float (&r)[D + 2]) const {
         (void)e;
@@ -866,37 +884,44 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
         res[k] = 0.f;
     }
     const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
-    int gidx[kMaxFaces], ngf = 0;  // the cell's boundary ghosts
-    auto add = [&](const int key, const float (&fr)[D + 2]) {
-        const int g = fa.ghost(key);
-        if (g >= 0 && ngf < kMaxFaces) gidx[ngf++] = g;
+    unsigned gmask = 0;  // bit j: entry j leads to a boundary ghost
+    // entry j's flux added (valid: j is one of the cell's entries; an
+    // invalid one is selected away, not branched around, so that the loads
+    // of a group stay together)
+    auto add = [&](const int j, const bool valid, const int key, const float (&fr)[D + 2]) {
+        gmask |= valid && fa.ghost(key) >= 0 ? 1u << j : 0u;
         const float sg = key >= 0 ? 1.f : -1.f;
 #pragma unroll
         for (int k = 0; k < D + 2; k++) {
-            const float r = sg * fr[k];
-            if (RES) res[k] += r;
-            dq[k] += dt * r * vinv;
+            const float r = valid ? sg * fr[k] : 0.f;
+            if (RES) res[k] = valid ? res[k] + r : res[k];
+            dq[k] = valid ? dq[k] + dt * r * vinv : dq[k];
         }
     };
-    // the first faces' fluxes fetched together, then added in gather order
+    // the faces in gather order, kUnroll at a time with their loads issued
+    // together (a cell has at most kMaxFaces).  Within a group no branch: an
+    // entry past the cell's last reads the last entry again (list) or the
+    // slot's placeholder, and is not added.
     constexpr int U = FA::kUnroll;
-    int key[U];
-    float fr[U][D + 2];
     fa.begin_cell(c);
 #pragma unroll
-    for (int j = 0; j < U; j++) {
-        const int jj = j < ne ? j : ne - 1;
-        key[j] = fa.key(e0 + jj, jj, c);
-        fa.get(e0 + jj, jj, key[j], fr[j]);
-    }
+    for (int j0 = 0; j0 < kMaxFaces; j0 += U) {
+        if (j0 < ne) {
+            int key[U];
+            float fr[U][D + 2];
 #pragma unroll
-    for (int j = 0; j < U; j++)
-        if (j < ne) add(key[j], fr[j]);
-    for (int e = e0 + U; e < e0 + ne; e++) {
-        const int kj = fa.key_far(e, e - e0, c);
-        float f1[D + 2];
-        fa.get_far(e, e - e0, kj, f1);
-        add(kj, f1);
+            for (int u = 0; u < U; u++) {
+                const int j = j0 + u;
+                if (j < kMaxFaces) {
+                    const int e = e0 + (j < ne ? j : ne - 1);
+                    key[u] = fa.key(e, j, c);
+                    fa.get(e, j, key[u], fr[u]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (j0 + u < kMaxFaces) add(j0 + u, j0 + u < ne, key[u], fr[u]);
+        }
     }
     load_q32<D>(a, c, q);
     const float sgm = a.sigma[c];
@@ -908,8 +933,9 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
     }
     store_q32<D>(qout, a.ntot, c, q);
     // refresh this cell's boundary ghosts for the next stage
-    for (int i = 0; i < ngf; i++) {
-        const int g = gidx[i];
+    for (unsigned m = gmask; m; m &= m - 1) {
+        const int j = __builtin_ctz(m);
+        const int g = fa.ghost(fa.key(e0 + j, j, c));
         float qg[D + 2];
         if (!ghost_state32<D>(a, a.ghost_role[g], q, qg)) {
             if (qout == a.q) continue;
@@ -968,60 +994,43 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, const FA &f
     if constexpr (CFL >= 0) block_extremum<CFL>(cv, a.partial + lblk);
 }
 
-// compute_dt / compute_cfl (cfd_v0.cpp:2887-2942) of one cell in FP64 from
-// its FP32 deviations q: the sum over its faces in gather order, then dt_c
-// (MODE 0) or cfl_c (MODE 1), combined with the block's starting value so a
-// NaN never wins
+// compute_dt / compute_cfl (cfd_v0.cpp:2887-2942) of one cell from its FP32
+// deviations q: the face terms |m.S| (compute_cfl) or |m.S|/|S| (compute_dt)
+// summed in FP32 over the cell's faces in gather order (the update's own
+// list, whose entries and area vectors it has just read; the FP64 path keeps
+// the reference's slot order), then dt_c (MODE 0) or cfl_c (MODE 1) in FP64,
+// combined with the block's starting value so a NaN never wins
 template <int D, int MODE, class FA>
 __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const FA &fa, const int c, const float (&qc)[D + 2],
                                              double param) {
-    const double rho = a.rho_inf_d + (double)qc[0];
-    double m[D];
+    float m[D];
 #pragma unroll
-    for (int k = 0; k < D; k++) m[k] = a.m_inf_d[k] + (double)qc[k + 1];
-    double sum = 0.0;
-    auto add = [&](const double (&S)[D]) {
-        double fl = 0.0;
-        if (MODE == 0) {
-            double sM = 0.0;
-#pragma unroll
-            for (int k = 0; k < D; k++) sM += S[k] * S[k];
-            sM = sqrt(sM);
-#pragma unroll
-            for (int k = 0; k < D; k++) fl += m[k] * S[k] / sM;
-        } else {
-#pragma unroll
-            for (int k = 0; k < D; k++) fl += m[k] * S[k];
-        }
-        sum += fabs(fl);
-    };
-    // the sum over the cell's faces in gather order (the update's own list,
-    // whose entries and area vectors it has just read; the FP64 path keeps
-    // the reference's slot order); the first faces' area vectors fetched
-    // together
+    for (int k = 0; k < D; k++) m[k] = a.minf[k] + qc[k + 1];
     const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
-    constexpr int U = GatherUnroll<D>::value;
-    double Sj[U][D];
+    float sum = 0.f;
 #pragma unroll
-    for (int j = 0; j < U; j++) {
-        const int jj = j < ne ? j : ne - 1;
-        fa.area(e0 + jj, jj, c, Sj[j]);
-    }
+    for (int j = 0; j < kMaxFaces; j++) {
+        float S[D];
+        fa.area(e0 + (j < ne ? j : ne - 1), j, c, S);
+        float fl = 0.f;
 #pragma unroll
-    for (int j = 0; j < U; j++)
-        if (j < ne) add(Sj[j]);
-    for (int e = e0 + U; e < e0 + ne; e++) {
-        double S[D];
-        fa.area_far(e, e - e0, c, S);
-        add(S);
+        for (int k = 0; k < D; k++) fl += m[k] * S[k];
+        if (MODE == 0) {
+            float s2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; k++) s2 += S[k] * S[k];
+            fl *= rsq32(s2);
+        }
+        sum = j < ne ? sum + fabsf(fl) : sum;
     }
-    const double vol = a.vol_d[c];
+    const double rho = a.rho_inf_d + (double)qc[0];
+    const double vinv = (double)a.vinv[c];
     double v = MODE == 0 ? 100.0 : 0.0;
     if (MODE == 0) {
-        const double dtc = param * rho * 2.0 * vol / sum;
+        const double dtc = param * rho * 2.0 / (vinv * (double)sum);
         if (v > dtc) v = dtc;
     } else {
-        const double cfl = sum * ((1.0 / rho) * (1.0 / vol) * param);
+        const double cfl = (double)sum * ((1.0 / rho) * vinv * param);
         if (v < cfl) v = cfl;
     }
     return v;
@@ -1090,9 +1099,9 @@ template <int D>
 __global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
     bc_fill32_body<D>(a, LFM_TID(blockIdx.x));
 }
-template <int D, int GRAD, bool VISC, bool RG>
+template <int D, int GRAD, bool VISC, bool RG, bool SL>
 __global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
-    grad_vis32_body<D, GRAD, VISC, RG>(a, LFM_TID(blockIdx.x));
+    grad_vis32_body<D, GRAD, VISC, RG, SL>(a, LFM_TID(blockIdx.x));
 }
 // 3D M1 / M2 faces: 5 waves per SIMD fit in 96 VGPRs without spills (97
 // would allow 4): O-grid face kernel 1.107 -> 1.033 ms per step.  Forcing
@@ -1118,8 +1127,9 @@ __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B
 template <int D, int SOLVER>
 struct FluxSlots32 {
     const KArgs32 &a;
-    // LDS for the area vectors the flux gather reads, kept for the CFL/dt
-    // sums after the update ([U][D][256] per block; nullptr: read again)
+    // LDS for the slots' area vectors, oriented out of the cell, kept for
+    // the CFL/dt sums after the update ([U][D][256] per block; nullptr: not
+    // kept)
     float *keep_S;
     static constexpr int U = GatherUnroll<D>::value;  // the slots per cell
     // 3D: two faces' loads issued together, which leaves 71 VGPRs and 7
@@ -1134,14 +1144,9 @@ struct FluxSlots32 {
         load_q32<D>(a, c, qc);
         pc = prim32<D>(a, qc);
     }
-    // key: the face's other cell, ~other where the cell is the face's N
+    // key: the face's other cell, ~other where the cell is the face's N;
+    // from the slots up to U, from the face arrays beyond
     __device__ __forceinline__ int key(int e, int j, int c) const {
-        (void)e;
-        return a.sl_nb[(size_t)j * a.nc + c];
-    }
-    // entries after the unrolled ones (j known at run time): the slots up
-    // to U, the face arrays beyond
-    __device__ __forceinline__ int key_far(int e, int j, int c) const {
         if (j < U) return a.sl_nb[(size_t)j * a.nc + c];
         const int ent = a.g_ent[e];
         return ent >= 0 ? a.fN[ent & kFaceMask] : ~a.fP[~ent];
@@ -1163,32 +1168,12 @@ struct FluxSlots32 {
         face_flux32_states<D, SOLVER, false, false>(a, 0, 0, 0, qP, qN, sp, sn, S, w, GradGlobal32<D, false>{a}, 0,
                                                      0, r);
     }
-    // slot j (< U, a compile-time index where called)
-    __device__ __forceinline__ void get(int e, int j, int key, float (&r)[D + 2]) const {
-        (void)e;
-        float S[D];
-#pragma unroll
-        for (int k = 0; k < D; k++) S[k] = a.sl_S[((size_t)j * D + k) * a.nc + c_];
-        if (keep_S) {
-#pragma unroll
-            for (int k = 0; k < D; k++) keep_S[(j * D + k) * 256 + threadIdx.x] = S[k];
-        }
-        flux(key, S, a.sl_w[(size_t)j * a.nc + c_], r);
-    }
-    __device__ __forceinline__ void get_far(int e, int j, int key, float (&r)[D + 2]) const {
-        float S[D], w;
-        far_data(e, j, S, w);
-        flux(key, S, w, r);
-    }
-    __device__ __forceinline__ void far_data(int e, int j, float (&S)[D], float &w) const {
+    // area vector and weight of entry j
+    __device__ __forceinline__ void data(int e, int j, float (&S)[D], float &w) const {
         if (j < U) {
 #pragma unroll
             for (int k = 0; k < D; k++) S[k] = a.sl_S[((size_t)j * D + k) * a.nc + c_];
             w = a.sl_w[(size_t)j * a.nc + c_];
-            if (keep_S) {
-#pragma unroll
-                for (int k = 0; k < D; k++) keep_S[(j * D + k) * 256 + threadIdx.x] = S[k];
-            }
             return;
         }
         const int ent = a.g_ent[e];
@@ -1196,26 +1181,26 @@ struct FluxSlots32 {
         load_S32<D>(a.S, f, S);
         w = a.w[f];
     }
-    __device__ __forceinline__ void area(int e, int j, int c, double (&Sd)[D]) const {
-        const bool isP = key(e, j, c) >= 0;
-        float S[D];
-#pragma unroll
-        for (int k = 0; k < D; k++)
-            S[k] = keep_S ? keep_S[(j * D + k) * 256 + threadIdx.x] : a.sl_S[((size_t)j * D + k) * a.nc + c];
-#pragma unroll
-        for (int k = 0; k < D; k++) Sd[k] = isP ? (double)S[k] : -(double)S[k];
-    }
-    __device__ __forceinline__ void area_far(int e, int j, int c, double (&Sd)[D]) const {
-        const bool isP = key_far(e, j, c) >= 0;
+    __device__ __forceinline__ void get(int e, int j, int key, float (&r)[D + 2]) const {
         float S[D], w;
+        data(e, j, S, w);
+        if (keep_S && j < U) {
+#pragma unroll
+            for (int k = 0; k < D; k++) keep_S[(j * D + k) * 256 + threadIdx.x] = key >= 0 ? S[k] : -S[k];
+        }
+        flux(key, S, w, r);
+    }
+    __device__ __forceinline__ void area(int e, int j, int c, float (&S)[D]) const {
         if (keep_S && j < U) {
 #pragma unroll
             for (int k = 0; k < D; k++) S[k] = keep_S[(j * D + k) * 256 + threadIdx.x];
-        } else {
-            far_data(e, j, S, w);
+            return;
         }
+        float w;
+        data(e, j, S, w);
+        const bool isP = key(e, j, c) >= 0;
 #pragma unroll
-        for (int k = 0; k < D; k++) Sd[k] = isP ? (double)S[k] : -(double)S[k];
+        for (int k = 0; k < D; k++) S[k] = isP ? S[k] : -S[k];
     }
 };
 
@@ -1228,7 +1213,16 @@ __global__ void __launch_bounds__(256) build_slots32(KArgs32 a, int32_t *nb, flo
     if (c >= a.nc) return;
     const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
     constexpr int U = GatherUnroll<D>::value;
-    for (int j = 0; j < U && j < ne; j++) {
+    for (int j = 0; j < U; j++) {
+        if (j >= ne) {
+            // placeholder past the cell's last entry: the cell itself, zero
+            // area (read by unbranched gathers, never added)
+            nb[(size_t)j * a.nc + c] = c;
+#pragma unroll
+            for (int k = 0; k < D; k++) S[((size_t)j * D + k) * a.nc + c] = 0.f;
+            w[(size_t)j * a.nc + c] = 0.5f;
+            continue;
+        }
         const int ent = a.g_ent[e0 + j];
         const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
         nb[(size_t)j * a.nc + c] = ent >= 0 ? a.fN[f] : ~a.fP[f];

@@ -388,38 +388,60 @@ void Solver::end_step(bool own_dt) {
     if (prm.have_res) res_valid = true;
 }
 
-// the second state buffer and the face slots of the one-launch stages,
-// allocated on first use (under the set-up mutex, as a capture on another
-// thread requires) and the slots filled from the gather list on the
-// handle's stream; when memory cannot be had the step runs the separate
+// the per-cell face slots (each cell's first GatherUnroll gather entries:
+// other cell, area vector, weight), read by the one-launch stages and the
+// FP32 gradient kernel; allocated on first use (under the set-up mutex, as a
+// capture on another thread requires) and filled from the gather list on
+// the handle's stream.  When memory cannot be had the kernels read the
+// gather list and the face arrays, which gives the same bits.
+bool Solver::ensure_slots32() {
+    if (!fp32 || slots_failed) return false;
+    if (k32->sl_nb) return true;
+    const size_t U = L.D == 2 ? GatherUnroll<2>::value : GatherUnroll<3>::value;
+    const size_t nsl = U * (size_t)L.nc;
+    std::lock_guard<std::mutex> lock(device_setup_mutex());
+    void *p[3] = {nullptr, nullptr, nullptr};
+    const size_t bytes[3] = {nsl * sizeof(int32_t), nsl * L.D * sizeof(float), nsl * sizeof(float)};
+    for (int i = 0; i < 3; i++)
+        if (hipMalloc(&p[i], bytes[i]) != hipSuccess) {
+            (void)hipGetLastError();
+            for (int k = 0; k < i; k++) (void)hipFree(p[k]);
+            slots_failed = true;
+            return false;
+        }
+    for (int i = 0; i < 3; i++) {
+        allocs.push_back(p[i]);
+        device_bytes += (int64_t)bytes[i];
+    }
+    LFM_DISPATCH_D(this, launch_build_slots32<2>(this, (int32_t *)p[0], (float *)p[1], (float *)p[2]),
+                   launch_build_slots32<3>(this, (int32_t *)p[0], (float *)p[1], (float *)p[2]));
+    k32->sl_nb = (const int32_t *)p[0];
+    k32->sl_S = (const float *)p[1];
+    k32->sl_w = (const float *)p[2];
+    return true;
+}
+
+// the second state buffer of the one-launch stages (and the slots they
+// read), allocated on first use; without them the step runs the separate
 // launches, which compute the same bits
 bool Solver::ensure_alt32() {
     if (!fused_ok) return false;
     if (q_alt32) return true;
-    const size_t nq = (size_t)(L.D + 2) * ((size_t)L.nc + L.ng);
-    const size_t U = L.D == 2 ? GatherUnroll<2>::value : GatherUnroll<3>::value;
-    const size_t nsl = U * (size_t)L.nc;
-    std::lock_guard<std::mutex> lock(device_setup_mutex());
-    void *p[4] = {nullptr, nullptr, nullptr, nullptr};
-    const size_t bytes[4] = {nq * sizeof(float), nsl * sizeof(int32_t), nsl * L.D * sizeof(float),
-                             nsl * sizeof(float)};
-    for (int i = 0; i < 4; i++)
-        if (hipMalloc(&p[i], bytes[i]) != hipSuccess) {
-            (void)hipGetLastError();
-            for (int k = 0; k < i; k++) (void)hipFree(p[k]);
-            fused_ok = false;
-            return false;
-        }
-    for (int i = 0; i < 4; i++) {
-        allocs.push_back(p[i]);
-        device_bytes += (int64_t)bytes[i];
+    if (!ensure_slots32()) {
+        fused_ok = false;
+        return false;
     }
-    q_alt32 = (float *)p[0];
-    LFM_DISPATCH_D(this, launch_build_slots32<2>(this, (int32_t *)p[1], (float *)p[2], (float *)p[3]),
-                   launch_build_slots32<3>(this, (int32_t *)p[1], (float *)p[2], (float *)p[3]));
-    k32->sl_nb = (const int32_t *)p[1];
-    k32->sl_S = (const float *)p[2];
-    k32->sl_w = (const float *)p[3];
+    const size_t nq = (size_t)(L.D + 2) * ((size_t)L.nc + L.ng);
+    std::lock_guard<std::mutex> lock(device_setup_mutex());
+    void *p = nullptr;
+    if (hipMalloc(&p, nq * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        fused_ok = false;
+        return false;
+    }
+    allocs.push_back(p);
+    device_bytes += (int64_t)(nq * sizeof(float));
+    q_alt32 = (float *)p;
     return true;
 }
 

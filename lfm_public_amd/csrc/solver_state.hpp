@@ -155,6 +155,9 @@ struct Solver {
     bool fused_ok = false;
     float *q_alt32 = nullptr;
     bool ensure_alt32();
+    // per-cell face slots of the FP32 kernels (k32->sl_*), on first use
+    bool slots_failed = false;
+    bool ensure_slots32();
     int fused_from() const;
     void stage_fused(int rk, bool from_alt);
     void end_step(bool own_dt);
