@@ -121,6 +121,37 @@ __host__ __device__ __forceinline__ size_t q32_index(int k, size_t i, size_t nto
     return k < 4 ? 4 * i + (size_t)k : 4 * ntot + i;
 }
 
+// loads and stores of data a kernel touches once, marked non-temporal so
+// that they do not displace the gathered neighbour states in L2: level 1 the
+// face slots, 2 the RK register, 3 the other streams (new states, face
+// lists and geometry of the face kernel, face fluxes, gradient records,
+// 1/V and sponge).  3D uses every level (256^3 box 3.70 -> 3.35 ms per
+// step, 4.2 M-cell O-grid 2.75 -> 2.51).  2D uses level 1 only: a 2D
+// solve's arrays fit in the 256 MB Infinity Cache, where the next kernel
+// finds what this one wrote unless it was written non-temporally (1 M-cell
+// annulus 0.32 -> 0.40 ms per step with every level).
+template <int D>
+constexpr int kStreamNT = D == 3 ? 3 : 1;
+using f4v = float __attribute__((ext_vector_type(4)));
+template <int D, int LEVEL, class T>
+__device__ __forceinline__ T ld_once(const T *p) {
+    if constexpr (kStreamNT<D> >= LEVEL) return __builtin_nontemporal_load(p);
+    return *p;
+}
+template <int D, int LEVEL, class T>
+__device__ __forceinline__ void st_once(T *p, T v) {
+    if constexpr (kStreamNT<D> >= LEVEL)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+// a 16-byte record
+template <int D, int LEVEL>
+__device__ __forceinline__ void st4_once(float *p, float x, float y, float z, float w) {
+    const f4v v = {x, y, z, w};
+    st_once<D, LEVEL>(reinterpret_cast<f4v *>(p), v);
+}
+
 template <int D>
 __device__ __forceinline__ void load_q32(const KArgs32 &a, int i, float *q) {
     const float4 v = reinterpret_cast<const float4 *>(a.q)[i];
@@ -133,8 +164,8 @@ __device__ __forceinline__ void load_q32(const KArgs32 &a, int i, float *q) {
 
 template <int D>
 __device__ __forceinline__ void store_q32(float *qv, size_t ntot, int i, const float *q) {
-    reinterpret_cast<float4 *>(qv)[i] = make_float4(q[0], q[1], q[2], q[3]);
-    if (D == 3) qv[4 * ntot + i] = q[4];
+    st4_once<D, 3>(qv + 4 * (size_t)i, q[0], q[1], q[2], q[3]);
+    if (D == 3) st_once<D, 3>(qv + 4 * ntot + i, q[4]);
 }
 
 template <int D>
@@ -327,12 +358,12 @@ template <int D, bool SL>
 __device__ __forceinline__ void gather_entry32(const KArgs32 &a, const int c, const int e, const int j, bool &isP,
                                                int &other, float &w, float (&S)[D]) {
     if (SL && j < GatherUnroll<D>::value) {
-        const int key = a.sl_nb[(size_t)j * a.nc + c];
+        const int key = ld_once<D, 1>(a.sl_nb + (size_t)j * a.nc + c);
         isP = key >= 0;
         other = isP ? key : ~key;
-        w = a.sl_w[(size_t)j * a.nc + c];
+        w = ld_once<D, 1>(a.sl_w + (size_t)j * a.nc + c);
 #pragma unroll
-        for (int k = 0; k < D; k++) S[k] = a.sl_S[((size_t)j * D + k) * a.nc + c];
+        for (int k = 0; k < D; k++) S[k] = ld_once<D, 1>(a.sl_S + ((size_t)j * D + k) * a.nc + c);
         return;
     }
     const int ent = a.g_ent[e];
@@ -492,7 +523,7 @@ __device__ __forceinline__ void grad_vis32_body(const KArgs32 &a, const int c) {
         float *dst = a.dudx + (size_t)c0 * S;
         const int n = nrec * S, n4 = n / 4;
         for (int j = t; j < n4; j += blockDim.x)
-            reinterpret_cast<float4 *>(dst)[j] = reinterpret_cast<const float4 *>(st)[j];
+            st_once<D, 3>(reinterpret_cast<f4v *>(dst) + j, reinterpret_cast<const f4v *>(st)[j]);
         for (int j = 4 * n4 + t; j < n; j += blockDim.x) dst[j] = st[j];
     }
 }
@@ -544,7 +575,7 @@ __device__ __forceinline__ void face_flux32_states(const KArgs32 &a, const int f
     const size_t nt = a.ntot;
     float d[D];
 #pragma unroll
-    for (int k = 0; k < D; k++) d[k] = a.d[(size_t)k * a.nf + f];
+    for (int k = 0; k < D; k++) d[k] = ld_once<D, 3>(a.d + (size_t)k * a.nf + f);
     float Sm2 = 0.f;
 #pragma unroll
     for (int k = 0; k < D; k++) Sm2 += S[k] * S[k];
@@ -796,14 +827,16 @@ __device__ __forceinline__ void face_flux32_states(const KArgs32 &a, const int f
 template <int D, int SOLVER, bool VISC, bool STORED, class GA>
 __device__ __forceinline__ void face_flux32_rhs(const KArgs32 &a, const int f, const GA &ga, const int gP,
                                                 const int gN, float (&rhs)[D + 2]) {
-    const int P = a.fP[f], N = a.fN[f];
+    const int P = ld_once<D, 3>(a.fP + f), N = ld_once<D, 3>(a.fN + f);
     float qP[D + 2], qN[D + 2];
     load_q32<D>(a, P, qP);
     load_q32<D>(a, N, qN);
     const Prim32<D> sp = prim32<D>(a, qP), sn = prim32<D>(a, qN);
     float S[D];
-    load_S32<D>(a.S, f, S);
-    face_flux32_states<D, SOLVER, VISC, STORED>(a, f, P, N, qP, qN, sp, sn, S, a.w[f], ga, gP, gN, rhs);
+#pragma unroll
+    for (int k = 0; k < D; k++) S[k] = ld_once<D, 3>(a.S + (size_t)f * D + k);
+    face_flux32_states<D, SOLVER, VISC, STORED>(a, f, P, N, qP, qN, sp, sn, S, ld_once<D, 3>(a.w + f), ga, gP, gN,
+                                                 rhs);
 }
 
 template <int D, int SOLVER, bool VISC, bool STORED>
@@ -811,8 +844,8 @@ __device__ __forceinline__ void face_flux32_body(const KArgs32 &a, const int f) 
     if (f >= a.nf) return;
     float rhs[D + 2];
     face_flux32_rhs<D, SOLVER, VISC, STORED>(a, f, GradGlobal32<D, SOLVER == 2>{a}, a.fP[f], a.fN[f], rhs);
-    reinterpret_cast<float4 *>(a.frhs)[f] = make_float4(rhs[0], rhs[1], rhs[2], rhs[3]);
-    if (D == 3) a.frhs[(size_t)4 * a.nf + f] = rhs[D + 1];
+    st4_once<D, 3>(a.frhs + 4 * (size_t)f, rhs[0], rhs[1], rhs[2], rhs[3]);
+    if (D == 3) st_once<D, 3>(a.frhs + (size_t)4 * a.nf + f, rhs[D + 1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -876,11 +909,11 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
                                                    const int c, float *qout, float (&res)[D + 2],
                                                    float (&q)[D + 2]) {
     const float dt = (float)(*a.dt);
-    const float vinv = a.vinv[c];
+    const float vinv = ld_once<D, 3>(a.vinv + c);
     float dq[D + 2];
 #pragma unroll
     for (int k = 0; k < D + 2; k++) {
-        dq[k] = zero_dq ? 0.f : a.dq[(size_t)k * a.nc + c] * A;
+        dq[k] = zero_dq ? 0.f : ld_once<D, 2>(a.dq + (size_t)k * a.nc + c) * A;
         res[k] = 0.f;
     }
     const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
@@ -924,12 +957,12 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
         }
     }
     load_q32<D>(a, c, q);
-    const float sgm = a.sigma[c];
+    const float sgm = ld_once<D, 3>(a.sigma + c);
 #pragma unroll
     for (int k = 0; k < D + 2; k++) {
         dq[k] -= dt * sgm * q[k];  // sponge: sigma (q_inf - q) = -sigma q'
         q[k] += B * dq[k];
-        a.dq[(size_t)k * a.nc + c] = dq[k];
+        st_once<D, 2>(a.dq + (size_t)k * a.nc + c, dq[k]);
     }
     store_q32<D>(qout, a.ntot, c, q);
     // refresh this cell's boundary ghosts for the next stage
@@ -1147,7 +1180,7 @@ struct FluxSlots32 {
     // key: the face's other cell, ~other where the cell is the face's N;
     // from the slots up to U, from the face arrays beyond
     __device__ __forceinline__ int key(int e, int j, int c) const {
-        if (j < U) return a.sl_nb[(size_t)j * a.nc + c];
+        if (j < U) return ld_once<D, 1>(a.sl_nb + (size_t)j * a.nc + c);
         const int ent = a.g_ent[e];
         return ent >= 0 ? a.fN[ent & kFaceMask] : ~a.fP[~ent];
     }
@@ -1172,8 +1205,8 @@ struct FluxSlots32 {
     __device__ __forceinline__ void data(int e, int j, float (&S)[D], float &w) const {
         if (j < U) {
 #pragma unroll
-            for (int k = 0; k < D; k++) S[k] = a.sl_S[((size_t)j * D + k) * a.nc + c_];
-            w = a.sl_w[(size_t)j * a.nc + c_];
+            for (int k = 0; k < D; k++) S[k] = ld_once<D, 1>(a.sl_S + ((size_t)j * D + k) * a.nc + c_);
+            w = ld_once<D, 1>(a.sl_w + (size_t)j * a.nc + c_);
             return;
         }
         const int ent = a.g_ent[e];
