@@ -89,7 +89,8 @@ def launch_plan(prm, nstages):
     {group: (launches, kernel name)}.  FP32 inviscid M1/M2 runs every stage
     (but a residual stage 0) as one stage_fused32 launch; other paths run
     grad_vis (when it has work), face_flux and cell_update per stage.  The
-    last stage's launch also forms the CFL/dt block extrema."""
+    last stage's launch also forms the CFL/dt block extrema, except on the
+    FP64 3D path, where they keep a launch of their own."""
     D, S = prm["dim"], prm["solver"]
     fp32 = prm["precision"] == 32
     visc = prm["mu"] != 0.0 or not prm.get("laminar", 1)
@@ -103,12 +104,18 @@ def launch_plan(prm, nstages):
             plan["face_flux"] = (first, f"face_flux32<{D}, {S}, false, false>")
         return plan
     rg = S == 2 and prm.get("minmod")
-    plan = {"face_flux": (nstages, f"face_flux{sfx}<{D}, {S}, {str(visc).lower()}, false>"),
-            "cell_update": (nstages - 1, f"cell_update{sfx}<{D}, false, -1>"),
-            "cell_update_cfl": (1, f"cell_update{sfx}<{D}, false, {mode}>")}
+    plan = {"face_flux": (nstages, f"face_flux{sfx}<{D}, {S}, {str(visc).lower()}, false>")}
+    if fp32 or D == 2:
+        plan["cell_update"] = (nstages - 1, f"cell_update{sfx}<{D}, false, -1>")
+        plan["cell_update_cfl"] = (1, f"cell_update{sfx}<{D}, false, {mode}>")
+    else:
+        plan["cell_update"] = (nstages, f"cell_update<{D}, false, -1>")
+        plan["cfl_dt"] = (1, f"cfl_dt_cells<{D}, {mode}>")
     if visc or S == 2:
+        # FP32 3D gradients read the face slots (the kernel's last parameter)
+        slots = f", {str(D == 3).lower()}" if fp32 else ""
         plan["grad_vis"] = (nstages, f"grad_vis{sfx}<{D}, {2 if rg else 0}, {str(visc).lower()}, "
-                                     f"{str(S == 2).lower()}>")
+                                     f"{str(S == 2).lower()}{slots}>")
     return plan
 
 
@@ -270,7 +277,7 @@ def roofline(case, nf, path_prm, nstages, prof, nprof, ms_step, w, traffic_key):
         out.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                     "algorithmic_bytes_per_launch": int(B),
                     "achieved_basis": f"one {kname} launch = one RK stage of every cell ({B} algorithmic bytes: "
-                                      f"SURVEY 8d per-stage formula) / its mean launch time "
+                                      f"Nc*[4(D+2)w+2w] + Nf*[(2D+1)w+8], w the word size) / its mean launch time "
                                       f"{per_launch[dom] * 1e3:.1f} us (HIP events, {nprof} steps)"})
     else:
         # three launches per stage: the per-stage bytes over the stage's time
