@@ -119,7 +119,30 @@ def launch_plan(prm, nstages):
     return plan
 
 
-def warm_up(g, min_steps, min_s, already_s=0.0):
+class Restarts:
+    """Keeps a solve within its configuration's own step count: before a run
+    of n steps would take it past `limit` steps from the initial state, the
+    initial state is loaded again.  Timing is independent of the state, but
+    a small case warmed up for seconds would otherwise march tens of
+    thousands of steps, past where its flow stays valid (the vortex leaves
+    the box)."""
+
+    def __init__(self, g, limit):
+        from lfm_public_amd import capi
+        self.g, self.limit, self.field = g, max(1, int(limit)), capi.FIELD_Q
+        self.q0 = g.q()
+        self.since = 0
+        self.count = 0
+
+    def before(self, n):
+        if self.since + n > self.limit:
+            self.g.set_field(self.field, self.q0)
+            self.since = 0
+            self.count += 1
+        self.since += n
+
+
+def warm_up(g, min_steps, min_s, already_s=0.0, restarts=None):
     """At least min_steps steps, and stepping until min_s seconds have passed
     including already_s of earlier stepping (GPU clocks settle); returns
     (steps, seconds of this call)."""
@@ -127,6 +150,9 @@ def warm_up(g, min_steps, min_s, already_s=0.0):
     n = 0
     chunk = max(1, min_steps)
     while True:
+        if restarts:
+            chunk = min(chunk, restarts.limit)
+            restarts.before(chunk)
         g.run(chunk)
         g.synchronize()
         n += chunk
@@ -137,23 +163,31 @@ def warm_up(g, min_steps, min_s, already_s=0.0):
         chunk = int(min(2000, max(1, (min_s - already_s - el) / max(per_step, 1e-6))))
 
 
-def time_path(case, precision, steps, warmup, warmup_s, nprof, err_steps=0):
+def time_path(case, precision, steps, warmup, warmup_s, nprof, err_steps=0, limit=None):
     """Create the solver; run err_steps steps from the initial state (the
     state then is kept for the FP32 error; the run also warms the GPU up);
-    warm up; time `steps` steps; profile nprof more by kernel group."""
+    warm up; time `steps` steps; profile nprof more by kernel group.  With a
+    step limit the warm-up, timed and profiled runs restart from the initial
+    state rather than go past it (a timed run longer than the limit runs
+    as it is)."""
     from lfm_public_amd import capi
     t0 = time.perf_counter()
     g = capi.GpuSolver.from_case(case, precision=precision)
     setup_s = time.perf_counter() - t0
     info = g.info()
+    restarts = Restarts(g, limit) if limit else None
     q_err, err_s = None, 0.0
     if err_steps > 0:
         t0 = time.perf_counter()
+        if restarts:
+            restarts.before(err_steps)
         g.run(err_steps)
         g.synchronize()
         err_s = time.perf_counter() - t0
         q_err = g.q()
-    wsteps, wsec = warm_up(g, warmup, warmup_s, err_s)
+    wsteps, wsec = warm_up(g, warmup, warmup_s, err_s, restarts)
+    if restarts:
+        restarts.before(steps)
     # timed region: the device drained on both sides, K steps queued back to
     # back; HIP events on the solver's stream bracket the same K steps
     g.synchronize()
@@ -161,10 +195,12 @@ def time_path(case, precision, steps, warmup, warmup_s, nprof, err_steps=0):
     ms_events = g.time_steps(steps)
     g.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3
+    if restarts:
+        restarts.before(nprof)
     prof, prof_total = g.profile_steps(nprof)
     return g, info, {"wall_ms": wall_ms, "event_ms": ms_events, "warmup_steps": wsteps,
                      "warmup_s": wsec + err_s, "error_run_steps": err_steps, "setup_s": setup_s, "prof": prof,
-                     "prof_total_ms": prof_total}, q_err
+                     "prof_total_ms": prof_total, "restarts": restarts.count if restarts else 0}, q_err
 
 
 def rel_errors(case, q, qref, D):
@@ -349,7 +385,7 @@ def main():
     q64 = None
     if have64:
         g64, info64, t64, q64 = time_path(case, capi.PREC_FP64, args.steps, args.warmup, args.warmup_s, nprof,
-                                          n_err)
+                                          n_err, limit=OWN_STEPS[args.workload])
         nf = info64.n_faces
         ms64 = t64["wall_ms"] / args.steps
         rate64 = nc * args.steps / (t64["wall_ms"] * 1e-3)
@@ -366,7 +402,7 @@ def main():
     if not args.fp64_only:
         try:
             g32, info32, t32, q32 = time_path(case, capi.PREC_FP32P, args.steps, args.warmup, args.warmup_s,
-                                              nprof, n_err)
+                                              nprof, n_err, limit=OWN_STEPS[args.workload])
             have32 = True
         except capi.LfmError as e:
             log(f"FP32-perturbation path unavailable: {e}")
@@ -416,6 +452,7 @@ def main():
     def path_rec(t, rate, ms, info):
         return {"cell_updates_per_s": rate, "ms_per_step": ms, "ms_per_step_events": t["event_ms"] / args.steps,
                 "warmup_steps": t["warmup_steps"], "warmup_s": round(t["warmup_s"], 2),
+                "restarts_from_initial_state": t["restarts"],
                 "setup_s": round(t["setup_s"], 1), "device_bytes": int(info.device_bytes),
                 "profiled_ms_per_step": t["prof_total_ms"] / nprof}
 
@@ -456,7 +493,7 @@ def secondary_run(workload, solver, steps=200):
     log(f"secondary {case.name}: {case.n_cells} cells")
     rec = {"workload": f"{case.name}: {desc}", "cells": case.n_cells, "steps": steps}
     for name, prec in (("fp32p", capi.PREC_FP32P), ("fp64", capi.PREC_FP64)):
-        g, info, t, _ = time_path(case, prec, steps, 20, 1.0, 1)
+        g, info, t, _ = time_path(case, prec, steps, 20, 1.0, 1, limit=OWN_STEPS[workload])
         g.close()
         ms = t["wall_ms"] / steps
         rec[name] = {"ms_per_step": ms, "cell_updates_per_s": case.n_cells / (ms * 1e-3)}

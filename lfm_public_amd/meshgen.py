@@ -126,18 +126,19 @@ def extrude_2d(nodes_xy: np.ndarray, cells: np.ndarray, classify: Callable[[np.n
                patch_names: Sequence[tuple], thickness: float = 1.0) -> PolyMesh:
     """Build a one-cell-thick polyMesh from planar cells.
 
-    cells: (Nc, 4) int array of CCW node ids, ``-1`` in column 3 for triangles.
+    cells: (Nc, K) int array of CCW node ids, K >= 4, padded with ``-1`` after
+    each cell's last node (triangles, quads, and polygons of up to K nodes).
     classify(mid_xy, edge_dir) -> patch index per boundary edge (into patch_names,
     which lists (name, type) pairs; an ``empty`` frontAndBack patch is appended).
     """
     nn = nodes_xy.shape[0]
-    nc = cells.shape[0]
-    nv = np.where(cells[:, 3] >= 0, 4, 3)
+    nc, K = cells.shape
+    nv = (cells >= 0).sum(axis=1)
     # edges a->b in CCW order of each cell
     a_list, b_list, c_list = [], [], []
-    for k in range(4):
+    for k in range(K):
         a = cells[:, k]
-        b = np.where(k + 1 < nv, cells[:, (k + 1) % 4], cells[:, 0])
+        b = np.where(k + 1 < nv, cells[:, (k + 1) % K], cells[:, 0])
         use = k < nv
         a_list.append(a[use]); b_list.append(b[use]); c_list.append(np.nonzero(use)[0])
     ea = np.concatenate(a_list).astype(np.int64)
@@ -183,15 +184,11 @@ def extrude_2d(nodes_xy: np.ndarray, cells: np.ndarray, classify: Callable[[np.n
         faces.append(side(fa_b[sel], fb_b[sel]))
         owner.append(own_b[sel])
         specs.append((name, ptype, int(sel.sum()), None))
-    # front/back: bottom faces reversed (outward -z), top faces CCW (outward +z)
-    q = cells[nv == 4]
-    t = cells[nv == 3][:, :3]
-    qi = np.nonzero(nv == 4)[0]
-    ti = np.nonzero(nv == 3)[0]
-    # keep front/back faces in cell order: emit per cell, quads and tris as two blocks
-    # (OpenFOAM only requires patch-contiguity)
-    faces += [q[:, ::-1], t[:, ::-1], q + nn, t + nn]
-    owner += [qi, ti, qi, ti]
+    # front/back: bottom faces reversed (outward -z), top faces CCW (outward
+    # +z), one block per node count (OpenFOAM only requires patch-contiguity)
+    groups = [(cells[nv == m][:, :m], np.nonzero(nv == m)[0]) for m in sorted(set(nv.tolist()), reverse=True)]
+    faces += [g[:, ::-1] for g, _ in groups] + [g + nn for g, _ in groups]
+    owner += [i for _, i in groups] + [i for _, i in groups]
     specs.append(("frontAndBack", "empty", 2 * nc, None))
     owner = np.concatenate(owner)
     return _assemble(pts, faces, owner, nei_i, own_i.shape[0], specs, nc, 2)
@@ -209,6 +206,53 @@ def cartesian_2d(nx: int, ny: int, x0=-10.0, x1=10.0, y0=-10.0, y1=10.0,
     n0 = j * (nx + 1) + i
     cells = np.stack([n0, n0 + 1, n0 + nx + 2, n0 + nx + 1], axis=-1).reshape(-1, 4)
     tol = 1e-9 * max(x1 - x0, y1 - y0)
+
+    def classify(mid, _dir):
+        pid = np.full(mid.shape[0], 0)
+        pid[np.abs(mid[:, 0] - x0) < tol] = 0
+        pid[np.abs(mid[:, 0] - x1) < tol] = 1
+        pid[np.abs(mid[:, 1] - y0) < tol] = 2
+        pid[np.abs(mid[:, 1] - y1) < tol] = 3
+        return pid
+
+    return extrude_2d(nodes, cells, classify, names, thickness)
+
+
+def polygon_strips_2d(nx: int, ny: int, h: float = 1.0, x0: float = 0.0, y0: float = 0.0,
+                      names=(("inlet", "patch"), ("outlet", "patch"), ("Inflow", "patch"), ("inflow", "patch")),
+                      thickness=1.0) -> PolyMesh:
+    """Columns of alternating coarse and fine cells on [x0, x0+nx h] x [y0, y0+ny h]:
+    even columns hold ny cells of height h, odd columns 2 ny cells of height
+    h/2, so a coarse cell meets two fine cells on each side it shares with a
+    fine column: 6 faces inside (hexagons with collinear edges), 5 in the
+    first column (its boundary side is one face) and in a last coarse column.
+    Patches x-, x+, y-, y+ in that order."""
+    ys = y0 + 0.5 * h * np.arange(2 * ny + 1)          # every vertical line carries the fine spacing
+    xs = x0 + h * np.arange(nx + 1)
+    # node (i, k) on vertical line i at ys[k]; the boundary lines of coarse
+    # columns with no fine neighbour on that side keep only the coarse nodes
+    keep = np.ones((nx + 1, 2 * ny + 1), bool)
+    for i in (0, nx):
+        col = 0 if i == 0 else nx - 1
+        if col % 2 == 0:
+            keep[i, 1::2] = False
+    idx = -np.ones(keep.shape, np.int64)
+    idx[keep] = np.arange(int(keep.sum()))
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    nodes = np.stack([X[keep], Y[keep]], axis=1)
+    cells = []
+    for i in range(nx):
+        if i % 2 == 0:
+            for j in range(ny):
+                k0, km, k1 = 2 * j, 2 * j + 1, 2 * j + 2
+                ring = [idx[i, k0], idx[i + 1, k0], idx[i + 1, km], idx[i + 1, k1], idx[i, k1], idx[i, km]]
+                cells.append([v for v in ring if v >= 0] + [-1] * sum(v < 0 for v in ring))
+        else:
+            for k in range(2 * ny):
+                cells.append([idx[i, k], idx[i + 1, k], idx[i + 1, k + 1], idx[i, k + 1], -1, -1])
+    cells = np.array(cells, np.int64)
+    x1, y1 = xs[-1], ys[-1]
+    tol = 1e-9 * h
 
     def classify(mid, _dir):
         pid = np.full(mid.shape[0], 0)

@@ -26,6 +26,8 @@ CASES = {
     # rkOrder 0: Mesh::solve runs no stage (mesh_solver.cpp:500); the state
     # stays and only the end-of-step reduction runs
     "vortex_rk0": lambda: sc.vortex(8, solver=0, rk_order=0),
+    # 2D cells of 5 and 6 faces
+    "poly_M2AUSM_visc_adjust": lambda: sc.PARITY_CASES["poly_M2AUSM_visc_adjust"](),
 }
 
 

@@ -4,9 +4,9 @@ the separate face-flux and update launches of the ISolver-shaped entry
 points (Mesh::solve order, mesh_solver.cpp:474-740): the state after each
 step must be identical bit for bit, since every face flux is the same
 function of the same inputs and every cell adds them in the same order.
-Covers 2D and 3D, M1 and M2, RK orders 1-5 (odd counts run their first stage
-in place), the residual stage, wall / inlet / outlet ghosts and an unset
-patch whose ghosts never change.
+Covers 2D and 3D, M1 and M2, RK orders 1-5, the residual stage, the
+adjusted dt, wall / inlet / outlet ghosts, an unset patch whose ghosts
+never change, and 2D cells of 5 and 6 faces.
 """
 import copy
 
@@ -48,6 +48,9 @@ CASES = {
     # adjustTimeStep: the last fused launch forms the next step's deltaT
     "vortex_M1_adjust": lambda: sc.vortex(16, solver=0, adjust_dt=1, max_co=0.3),
     "box_M2_adjust_rk3": lambda: sc.seeded_perturbation(sc.box(6, solver=1, rk_order=3, adjust_dt=1, max_co=0.3)),
+    # 2D cells of 5 and 6 faces: gather entries past the face slots
+    "poly_M1": lambda: sc.poly(solver=0),
+    "poly_M2_adjust": lambda: sc.poly(solver=1, adjust_dt=1, max_co=0.3),
 }
 
 

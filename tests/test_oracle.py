@@ -262,6 +262,57 @@ def test_rk_amplification_matches_taylor(O, rk):
     assert np.allclose(coef[: len(want)], want, rtol=1e-7, atol=1e-8), coef
 
 
+def test_polygon_strips_geometry_and_invariants(O):
+    """2D cells of 4, 5 and 6 faces (coarse columns meeting two fine cells per
+    side): volumes, centres and face closure in closed form, then freestream
+    preservation and, with every patch a wall, conservation of mass and
+    energy for each scheme."""
+    nx, ny, h, th = 7, 4, 1.0, 1.0
+    m = meshgen.polygon_strips_2d(nx, ny, h=h, thickness=th)
+    case = sc.poly(nx, ny, solver=0)
+    o = O.Oracle(m, case.params, np.full(m.n_cells, 1e5), np.full(m.n_cells, 300.0), np.zeros((m.n_cells, 3)))
+    vol = o.field(O.F_VOLUME)[0]
+    cc = o.field(O.F_CENTRE)
+    nface = o.slots()[0]
+    S = o.slots()[3]
+    c = 0
+    for i in range(nx):
+        coarse = i % 2 == 0
+        for j in range(ny if coarse else 2 * ny):
+            hy = h if coarse else 0.5 * h
+            assert vol[c] == pytest.approx(h * hy * th, rel=1e-13)
+            assert cc[0][c] == pytest.approx((i + 0.5) * h, abs=1e-12)
+            assert cc[1][c] == pytest.approx((j + 0.5) * hy, abs=1e-12)
+            expect = 4 if not coarse else (5 if i in (0, nx - 1) else 6)
+            assert nface[c] == expect, (c, i, j)
+            c += 1
+    assert c == m.n_cells
+    assert np.abs(S.sum(axis=1)[:, :2]).max() < 1e-12
+    for solver in (0, 1, 2):
+        # freestream: uniform flow stays uniform (inlet-type patches all round)
+        fs = sc.poly(nx, ny, solver=solver, minmod=1, mu=0.01)
+        U = np.zeros((fs.n_cells, 3))
+        U[:, 0] = fs.params["U_inf"][0]
+        of = O.Oracle(fs.mesh, fs.params, np.full(fs.n_cells, fs.params["p_inf"]),
+                      np.full(fs.n_cells, fs.params["T_inf"]), U)
+        q0 = of.q()
+        of.run(3)
+        dq = np.abs(of.q() - q0)
+        assert dq[0].max() < 1e-13 and dq[-1].max() < 1e-12 and dq[1:-1].max() < 1e-13
+        # all walls: mass and energy conserved for a perturbed state
+        wc = sc.poly(nx, ny, solver=solver, minmod=1, mu=2e-3)
+        wm = wc.mesh
+        wm.patches = [meshgen.Patch(p.name, "wall" if p.type != "empty" else "empty", p.start, p.size)
+                      for p in wm.patches]
+        ow = O.Oracle(wm, wc.params, wc.p, wc.T, wc.U)
+        vw = ow.field(O.F_VOLUME)[0]
+        t0 = (ow.q() * vw).sum(axis=1)
+        ow.run(10)
+        t1 = (ow.q() * vw).sum(axis=1)
+        assert abs(t1[0] - t0[0]) <= 1e-13 * t0[0]
+        assert abs(t1[-1] - t0[-1]) <= 1e-13 * t0[-1]
+
+
 def test_closed_box_conserves_mass_and_energy(O):
     """All-wall box: wall ghosts mirror the velocity and keep rho, rhoE, so
     no mass or energy crosses the walls; the owner/neighbour scatter is
