@@ -114,7 +114,10 @@ typedef struct {
 typedef struct {
     int32_t dim, n_cells, n_cells_sm0, n_faces, n_ghosts, max_faces_per_cell;
     int32_t precision, solver;
-    int64_t device_bytes;   /* device memory held by the handle            */
+    int64_t device_bytes;   /* device memory held by the handle; an FP32P
+                               handle adds its face slots (3D gradients, or
+                               inviscid M1/M2) and second state buffer
+                               (inviscid M1/M2) when it first uses them   */
     double dt, cfl;         /* current deltaT and last convective CFL      */
     int64_t steps_done;
 } lfm_info;

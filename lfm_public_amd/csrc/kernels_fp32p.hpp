@@ -32,7 +32,8 @@ struct KArgs32 {
     const float *d, *w;
     const int32_t *g_off, *g_ent;
     const int32_t *ghost_cell, *ghost_role;
-    const float *vinv, *sigma;
+    const float *vinv;
+    const float *sigma;  // nullptr where every cell's sponge sigma is zero
     const float *smag;  // [nc] Smagorinsky constants, nullptr when laminar
     float *q, *dq, *res_cell;
     // per-cell gradient records (GradRec32): dudx points at the records,
@@ -350,6 +351,20 @@ struct UpdateUnroll32 {
     static constexpr int value = GatherUnroll<D>::value;
 };
 
+// Face slot storage: for each group of 64 consecutive cells (one wave's
+// cells), slot j's field f (NF fields per slot) at
+// [c / 64][j][f][c % 64], so that a wave reads one contiguous block of each
+// slot array instead of a separate strided stream per slot and field
+// (kSlotBlocked; otherwise [j][f][c]).  Arrays hold whole groups.
+constexpr bool kSlotBlocked = true;
+template <int D, int NF>
+__device__ __forceinline__ size_t slot_at(int nc, int j, int f, int c) {
+    if constexpr (kSlotBlocked)
+        return ((size_t)(c >> 6) * (GatherUnroll<D>::value * NF) + (size_t)(j * NF + f)) * 64 + (size_t)(c & 63);
+    return ((size_t)j * NF + f) * (size_t)nc + c;
+}
+inline size_t slot_array_len(int nc, int U, int NF) { return (size_t)(nc + 63) / 64 * 64 * U * NF; }
+
 // gather entry j of cell c (entry e of the list): its side, other cell,
 // weight and area vector.  SL: from the per-cell face slots for the first
 // GatherUnroll entries (coalesced [j][cell] arrays, no face index to chase);
@@ -358,12 +373,12 @@ template <int D, bool SL>
 __device__ __forceinline__ void gather_entry32(const KArgs32 &a, const int c, const int e, const int j, bool &isP,
                                                int &other, float &w, float (&S)[D]) {
     if (SL && j < GatherUnroll<D>::value) {
-        const int key = ld_once<D, 1>(a.sl_nb + (size_t)j * a.nc + c);
+        const int key = ld_once<D, 1>(a.sl_nb + slot_at<D, 1>(a.nc, j, 0, c));
         isP = key >= 0;
         other = isP ? key : ~key;
-        w = ld_once<D, 1>(a.sl_w + (size_t)j * a.nc + c);
+        w = ld_once<D, 1>(a.sl_w + slot_at<D, 1>(a.nc, j, 0, c));
 #pragma unroll
-        for (int k = 0; k < D; k++) S[k] = ld_once<D, 1>(a.sl_S + ((size_t)j * D + k) * a.nc + c);
+        for (int k = 0; k < D; k++) S[k] = ld_once<D, 1>(a.sl_S + slot_at<D, D>(a.nc, j, k, c));
         return;
     }
     const int ent = a.g_ent[e];
@@ -865,6 +880,8 @@ template <int D>
 struct FaceList32 {
     const KArgs32 &a;
     static constexpr int kUnroll = UpdateUnroll32<D>::value;
+    static constexpr bool kAllSlots = false;
+    __device__ __forceinline__ static bool valid(int key, int c) { return (void)key, (void)c, true; }
     __device__ __forceinline__ void begin_cell(int c) const { (void)c; }
     __device__ __forceinline__ int key(int e, int j, int c) const {
         (void)j;
@@ -916,7 +933,19 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
         dq[k] = zero_dq ? 0.f : ld_once<D, 2>(a.dq + (size_t)k * a.nc + c) * A;
         res[k] = 0.f;
     }
-    const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
+    // kAllSlots: every entry is in the face slots (placeholders past the
+    // last), so the gather list's offsets are not read; the group count is
+    // then kept opaque to the compiler, which would otherwise set up every
+    // group's addresses at once and spill
+    int e0, ne;
+    if constexpr (FA::kAllSlots) {
+        e0 = 0;
+        ne = kMaxFaces;
+        asm volatile("" : "+s"(ne));
+    } else {
+        e0 = a.g_off[c];
+        ne = a.g_off[c + 1] - e0;
+    }
     unsigned gmask = 0;  // bit j: entry j leads to a boundary ghost
     // entry j's flux added (valid: j is one of the cell's entries; an
     // invalid one is selected away, not branched around, so that the loads
@@ -953,14 +982,16 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
             }
 #pragma unroll
             for (int u = 0; u < U; u++)
-                if (j0 + u < kMaxFaces) add(j0 + u, j0 + u < ne, key[u], fr[u]);
+                if (j0 + u < kMaxFaces) add(j0 + u, FA::kAllSlots ? fa.valid(key[u], c) : j0 + u < ne, key[u], fr[u]);
         }
     }
     load_q32<D>(a, c, q);
-    const float sgm = ld_once<D, 3>(a.sigma + c);
+    // sponge: sigma (q_inf - q) = -sigma q' (no sigma array, and no term,
+    // where the case has no sponge layer and every sigma is zero)
+    const float sgm = a.sigma ? ld_once<D, 3>(a.sigma + c) : 0.f;
 #pragma unroll
     for (int k = 0; k < D + 2; k++) {
-        dq[k] -= dt * sgm * q[k];  // sponge: sigma (q_inf - q) = -sigma q'
+        if (a.sigma) dq[k] -= dt * sgm * q[k];
         q[k] += B * dq[k];
         st_once<D, 2>(a.dq + (size_t)k * a.nc + c, dq[k]);
     }
@@ -1039,22 +1070,22 @@ __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const FA &fa, con
     float m[D];
 #pragma unroll
     for (int k = 0; k < D; k++) m[k] = a.minf[k] + qc[k + 1];
-    const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
+    // kAllSlots: every entry in the slots, a placeholder known by its zero
+    // area vector (a face has a nonzero one)
+    const int e0 = FA::kAllSlots ? 0 : a.g_off[c];
+    const int ne = FA::kAllSlots ? kMaxFaces : a.g_off[c + 1] - e0;
     float sum = 0.f;
 #pragma unroll
     for (int j = 0; j < kMaxFaces; j++) {
         float S[D];
         fa.area(e0 + (j < ne ? j : ne - 1), j, c, S);
-        float fl = 0.f;
+        float fl = 0.f, s2 = 0.f;
 #pragma unroll
         for (int k = 0; k < D; k++) fl += m[k] * S[k];
-        if (MODE == 0) {
-            float s2 = 0.f;
 #pragma unroll
-            for (int k = 0; k < D; k++) s2 += S[k] * S[k];
-            fl *= rsq32(s2);
-        }
-        sum = j < ne ? sum + fabsf(fl) : sum;
+        for (int k = 0; k < D; k++) s2 += S[k] * S[k];
+        if (MODE == 0) fl *= rsq32(s2);
+        sum = (FA::kAllSlots ? s2 > 0.f : j < ne) ? sum + fabsf(fl) : sum;
     }
     const double rho = a.rho_inf_d + (double)qc[0];
     const double vinv = (double)a.vinv[c];
@@ -1169,6 +1200,10 @@ struct FluxSlots32 {
     // waves per SIMD (256^3 box: 0.82 -> 0.75 ms per stage against all six
     // at 4 waves)
     static constexpr int kUnroll = D == 3 ? 2 : U;
+    // a 3D cell's entries all fit the slots (kMaxFaces of them)
+    static constexpr bool kAllSlots = U >= kMaxFaces;
+    // a slot past the cell's last entry holds the cell itself
+    __device__ __forceinline__ static bool valid(int key, int c) { return key != c; }
     mutable float qc[D + 2];
     mutable Prim32<D> pc;
     mutable int c_ = 0;
@@ -1180,7 +1215,7 @@ struct FluxSlots32 {
     // key: the face's other cell, ~other where the cell is the face's N;
     // from the slots up to U, from the face arrays beyond
     __device__ __forceinline__ int key(int e, int j, int c) const {
-        if (j < U) return ld_once<D, 1>(a.sl_nb + (size_t)j * a.nc + c);
+        if (j < U) return ld_once<D, 1>(a.sl_nb + slot_at<D, 1>(a.nc, j, 0, c));
         const int ent = a.g_ent[e];
         return ent >= 0 ? a.fN[ent & kFaceMask] : ~a.fP[~ent];
     }
@@ -1205,8 +1240,8 @@ struct FluxSlots32 {
     __device__ __forceinline__ void data(int e, int j, float (&S)[D], float &w) const {
         if (j < U) {
 #pragma unroll
-            for (int k = 0; k < D; k++) S[k] = ld_once<D, 1>(a.sl_S + ((size_t)j * D + k) * a.nc + c_);
-            w = ld_once<D, 1>(a.sl_w + (size_t)j * a.nc + c_);
+            for (int k = 0; k < D; k++) S[k] = ld_once<D, 1>(a.sl_S + slot_at<D, D>(a.nc, j, k, c_));
+            w = ld_once<D, 1>(a.sl_w + slot_at<D, 1>(a.nc, j, 0, c_));
             return;
         }
         const int ent = a.g_ent[e];
@@ -1250,18 +1285,18 @@ __global__ void __launch_bounds__(256) build_slots32(KArgs32 a, int32_t *nb, flo
         if (j >= ne) {
             // placeholder past the cell's last entry: the cell itself, zero
             // area (read by unbranched gathers, never added)
-            nb[(size_t)j * a.nc + c] = c;
+            nb[slot_at<D, 1>(a.nc, j, 0, c)] = c;
 #pragma unroll
-            for (int k = 0; k < D; k++) S[((size_t)j * D + k) * a.nc + c] = 0.f;
-            w[(size_t)j * a.nc + c] = 0.5f;
+            for (int k = 0; k < D; k++) S[slot_at<D, D>(a.nc, j, k, c)] = 0.f;
+            w[slot_at<D, 1>(a.nc, j, 0, c)] = 0.5f;
             continue;
         }
         const int ent = a.g_ent[e0 + j];
         const int f = ent >= 0 ? (ent & kFaceMask) : ~ent;
-        nb[(size_t)j * a.nc + c] = ent >= 0 ? a.fN[f] : ~a.fP[f];
+        nb[slot_at<D, 1>(a.nc, j, 0, c)] = ent >= 0 ? a.fN[f] : ~a.fP[f];
 #pragma unroll
-        for (int k = 0; k < D; k++) S[((size_t)j * D + k) * a.nc + c] = a.S[(size_t)f * D + k];
-        w[(size_t)j * a.nc + c] = a.w[f];
+        for (int k = 0; k < D; k++) S[slot_at<D, D>(a.nc, j, k, c)] = a.S[(size_t)f * D + k];
+        w[slot_at<D, 1>(a.nc, j, 0, c)] = a.w[f];
     }
 }
 

@@ -397,11 +397,11 @@ void Solver::end_step(bool own_dt) {
 bool Solver::ensure_slots32() {
     if (!fp32 || slots_failed) return false;
     if (k32->sl_nb) return true;
-    const size_t U = L.D == 2 ? GatherUnroll<2>::value : GatherUnroll<3>::value;
-    const size_t nsl = U * (size_t)L.nc;
+    const int U = L.D == 2 ? GatherUnroll<2>::value : GatherUnroll<3>::value;
     std::lock_guard<std::mutex> lock(device_setup_mutex());
     void *p[3] = {nullptr, nullptr, nullptr};
-    const size_t bytes[3] = {nsl * sizeof(int32_t), nsl * L.D * sizeof(float), nsl * sizeof(float)};
+    const size_t bytes[3] = {slot_array_len(L.nc, U, 1) * sizeof(int32_t),
+                             slot_array_len(L.nc, U, L.D) * sizeof(float), slot_array_len(L.nc, U, 1) * sizeof(float)};
     for (int i = 0; i < 3; i++)
         if (hipMalloc(&p[i], bytes[i]) != hipSuccess) {
             (void)hipGetLastError();

@@ -94,7 +94,9 @@ int create_fp32p(Solver *s) {
         rc |= up(s, (float **)&a.w, w.data(), w.size());
         auto vinv = to_f32(L.vinv), sig = to_f32(L.sigma);
         rc |= up(s, (float **)&a.vinv, vinv.data(), nc);
-        rc |= up(s, (float **)&a.sigma, sig.data(), nc);
+        bool sponge = false;
+        for (float v : sig) sponge |= v != 0.f;
+        if (sponge) rc |= up(s, (float **)&a.sigma, sig.data(), nc);
         if (!L.smag.empty()) {
             auto sm = to_f32(L.smag);
             rc |= up(s, (float **)&a.smag, sm.data(), nc);
