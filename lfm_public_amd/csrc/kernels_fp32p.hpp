@@ -435,11 +435,14 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
             }
         }
     };
-    const int e0 = a.g_off[c], ne = a.g_off[c + 1] - e0;
+    // every entry in the slots (3D): the gather list's offsets are not read,
+    // a placeholder is known by its other cell, the cell itself
+    constexpr bool kAll = SL && GatherUnroll<D>::value >= kMaxFaces;
+    const int e0 = kAll ? 0 : a.g_off[c], ne = kAll ? kMaxFaces : a.g_off[c + 1] - e0;
     // the first faces: every load issued before the sums (a short list
     // repeats its last face, whose terms are then not added)
     constexpr int U = GradUnroll32<D>::value;
-    bool isP[U];
+    bool isP[U], valid[U];
     float w[U], S[U][D], qo[U][D + 2];
 #pragma unroll
     for (int j = 0; j < U; j++) {
@@ -448,11 +451,12 @@ __device__ __forceinline__ Prim32<D> grad32_sums(const KArgs32 &a, const int c, 
         const int jj = j < ne ? j : ne - 1;
         int other;
         gather_entry32<D, SL>(a, c, e0 + jj, SL ? j : jj, isP[j], other, w[j], S[j]);
+        valid[j] = kAll ? !(isP[j] && other == c) : j < ne;
         load_q32<D>(a, other, qo[j]);
     }
 #pragma unroll
     for (int j = 0; j < U; j++)
-        if (j < ne) add(isP[j], w[j], S[j], qo[j]);
+        if (valid[j]) add(isP[j], w[j], S[j], qo[j]);
     for (int e = e0 + U; e < e0 + ne; e++) {
         bool p;
         int other;
@@ -553,12 +557,25 @@ struct GradGlobal32 {
     using R = GradRec32<D, RG>;
     // the gradient record of cell c
     __device__ __forceinline__ void rec(int c, float (&r)[R::kStride]) const {
-        const float2 *p = reinterpret_cast<const float2 *>(a.dudx + (size_t)c * R::kStride);
+        if constexpr (R::kStride % 4 == 0) {
+            // records of whole 16-byte words (3D without minmod: 48 B)
+            const float4 *p = reinterpret_cast<const float4 *>(a.dudx + (size_t)c * R::kStride);
 #pragma unroll
-        for (int k = 0; k < R::kStride / 2; k++) {
-            const float2 v = p[k];
-            r[2 * k] = v.x;
-            r[2 * k + 1] = v.y;
+            for (int k = 0; k < R::kStride / 4; k++) {
+                const float4 v = p[k];
+                r[4 * k] = v.x;
+                r[4 * k + 1] = v.y;
+                r[4 * k + 2] = v.z;
+                r[4 * k + 3] = v.w;
+            }
+        } else {
+            const float2 *p = reinterpret_cast<const float2 *>(a.dudx + (size_t)c * R::kStride);
+#pragma unroll
+            for (int k = 0; k < R::kStride / 2; k++) {
+                const float2 v = p[k];
+                r[2 * k] = v.x;
+                r[2 * k + 1] = v.y;
+            }
         }
     }
     // U_grad: dudx itself, or its own [component][ntot] array
@@ -1164,7 +1181,8 @@ __global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
     bc_fill32_body<D>(a, LFM_TID(blockIdx.x));
 }
 template <int D, int GRAD, bool VISC, bool RG, bool SL>
-__global__ void __launch_bounds__(256) grad_vis32(KArgs32 a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 3 && SL && !RG ? 5 : 1)))
+grad_vis32(KArgs32 a) {
     grad_vis32_body<D, GRAD, VISC, RG, SL>(a, LFM_TID(blockIdx.x));
 }
 // 3D M1 / M2 faces: 5 waves per SIMD fit in 96 VGPRs without spills (97
