@@ -1214,9 +1214,8 @@ struct FluxSlots32 {
     // kept)
     float *keep_S;
     static constexpr int U = GatherUnroll<D>::value;  // the slots per cell
-    // 3D: two faces' loads issued together, which leaves 71 VGPRs and 7
-    // waves per SIMD (256^3 box: 0.82 -> 0.75 ms per stage against all six
-    // at 4 waves)
+    // 3D: two faces' loads issued together, which leaves 63 VGPRs and 8
+    // waves per SIMD (all six at 4 waves, or three, were slower)
     static constexpr int kUnroll = D == 3 ? 2 : U;
     // a 3D cell's entries all fit the slots (kMaxFaces of them)
     static constexpr bool kAllSlots = U >= kMaxFaces;
@@ -1321,8 +1320,9 @@ __global__ void __launch_bounds__(256) build_slots32(KArgs32 a, int32_t *nb, flo
 // One RK stage in one launch for inviscid M1 / M2: the face fluxes, the
 // ordered gather, sponge and update of cell_update32, reading the state in
 // a.q and writing the new state and boundary ghosts to qout (another buffer).
-// 3D: 7 waves per SIMD (6 for the last stage, whose CFL/dt sums keep more
-// live), with two faces' loads in flight per wave (kUnroll).
+// 3D: at least 7 waves per SIMD (6 for the last stage, whose CFL/dt sums
+// keep more live; both compile to 8), with two faces' loads in flight per
+// wave (kUnroll).
 template <int D, int SOLVER, int CFL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 3 ? (CFL < 0 ? 7 : 6) : 1)))
 stage_fused32(KArgs32 a, float *qout, float A, float B, int zero_dq, double cfl_param) {
