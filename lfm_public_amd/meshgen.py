@@ -420,6 +420,127 @@ def _hex_block(P: np.ndarray, ni: int, nj: int, nk: int, periodic_j: bool,
     return _assemble(pts, faces, own_all, N, O.shape[0], specs, ni * nj * nk, 3)
 
 
+def extrude_layers_3d(nodes_xy: np.ndarray, cells: np.ndarray, nz: int, dz: float,
+                      classify: Callable[[np.ndarray, np.ndarray], np.ndarray],
+                      patch_names: Sequence[tuple], bottom=("Outflow", "patch"),
+                      top=("outflow", "patch")) -> PolyMesh:
+    """A 3D mesh of nz layers of thickness dz over a planar mesh of triangles
+    and quads (cells as in extrude_2d): triangular prisms (5 faces) and hexes
+    (6).  Cell id = layer * n2 + planar cell.  Side patches come from
+    classify (as in extrude_2d); the bottom and top faces form the last two
+    patches."""
+    nn = nodes_xy.shape[0]
+    n2, K = cells.shape
+    nv = (cells >= 0).sum(axis=1)
+    a_l, b_l, c_l = [], [], []
+    for k in range(K):
+        a = cells[:, k]
+        b = np.where(k + 1 < nv, cells[:, (k + 1) % K], cells[:, 0])
+        use = k < nv
+        a_l.append(a[use]); b_l.append(b[use]); c_l.append(np.nonzero(use)[0])
+    ea, eb, ec = (np.concatenate(x).astype(np.int64) for x in (a_l, b_l, c_l))
+    key = np.minimum(ea, eb) * nn + np.maximum(ea, eb)
+    order = np.lexsort((ec, key))
+    key_s, ea_s, eb_s, ec_s = key[order], ea[order], eb[order], ec[order]
+    same = np.zeros(key_s.shape[0], bool)
+    same[:-1] = key_s[1:] == key_s[:-1]
+    first = np.nonzero(same)[0]
+    second = np.zeros_like(same)
+    second[first + 1] = True
+    bnd = np.nonzero(~same & ~second)[0]
+    # planar interior edges (owner = lower cell, along its CCW edge) and
+    # boundary edges with their patches
+    ie_o, ie_n, ie_a, ie_b = ec_s[first], ec_s[first + 1], ea_s[first], eb_s[first]
+    be_o, be_a, be_b = ec_s[bnd], ea_s[bnd], eb_s[bnd]
+    pid = classify(0.5 * (nodes_xy[be_a] + nodes_xy[be_b]), nodes_xy[be_b] - nodes_xy[be_a])
+
+    pts = np.zeros(((nz + 1) * nn, 3))
+    for k in range(nz + 1):
+        pts[k * nn:(k + 1) * nn, :2] = nodes_xy
+        pts[k * nn:(k + 1) * nn, 2] = k * dz
+
+    def side(a, b, k):
+        return np.stack([a + k * nn, b + k * nn, b + (k + 1) * nn, a + (k + 1) * nn], axis=1)
+
+    # internal faces: side faces of every layer and the faces between
+    # layers, as (owner, neighbour, points) then sorted upper-triangular
+    own, nei, fpts = [], [], []
+    for k in range(nz):
+        own.append(k * n2 + ie_o); nei.append(k * n2 + ie_n)
+        fpts += [list(r) for r in side(ie_a, ie_b, k)]
+    for k in range(nz - 1):
+        for i in range(n2):
+            own.append(np.array([k * n2 + i])); nei.append(np.array([(k + 1) * n2 + i]))
+            fpts.append(list(cells[i, :nv[i]] + (k + 1) * nn))
+    own = np.concatenate(own); nei = np.concatenate(nei)
+    srt = np.lexsort((nei, own))
+    faces = [np.array(fpts[f], np.int64) for f in srt]
+    owner = [own[srt]]
+    n_internal = len(faces)
+    specs = []
+    for p, (name, ptype) in enumerate(patch_names):
+        sel = np.nonzero(pid == p)[0]
+        cnt = 0
+        for k in range(nz):
+            for e in sel:
+                faces.append(side(be_a[e:e + 1], be_b[e:e + 1], k)[0])
+            owner.append(k * n2 + be_o[sel])
+            cnt += sel.shape[0]
+        specs.append((name, ptype, cnt, None))
+    for (name, ptype), k, rev in ((bottom, 0, True), (top, nz - 1, False)):
+        lvl = 0 if rev else nz
+        for i in range(n2):
+            ring = cells[i, :nv[i]] + lvl * nn
+            faces.append(ring[::-1] if rev else ring)
+        owner.append(k * n2 + np.arange(n2))
+        specs.append((name, ptype, n2, None))
+    owner = np.concatenate(owner)
+    sizes = np.array([len(f) for f in faces], np.int64)
+    face_off = np.zeros(len(faces) + 1, np.int64)
+    np.cumsum(sizes, out=face_off[1:])
+    patches, start = [], n_internal
+    for name, ptype, size, nbr in specs:
+        patches.append(Patch(name, ptype, start, int(size), nbr))
+        start += int(size)
+    return PolyMesh(np.ascontiguousarray(pts), face_off.astype(np.int32),
+                    np.concatenate(faces).astype(np.int32), owner.astype(np.int32),
+                    nei[srt].astype(np.int32), nz * n2, patches, 3)
+
+
+def prism_hex_3d(nx: int, ny: int, nz: int, h: float = 1.0) -> PolyMesh:
+    """nz layers over an nx x ny planar grid whose left half is quads and
+    right half triangles (each quad split along a diagonal): hexes and
+    triangular prisms.  Patches inlet (x-), outlet (x+), Inflow (y-), inflow
+    (y+), Outflow (bottom), outflow (top)."""
+    xs = np.arange(nx + 1) * h
+    ys = np.arange(ny + 1) * h
+    X, Y = np.meshgrid(xs, ys, indexing="xy")
+    nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+    cells = []
+    for j in range(ny):
+        for i in range(nx):
+            n0 = j * (nx + 1) + i
+            q = [n0, n0 + 1, n0 + nx + 2, n0 + nx + 1]
+            if i < nx // 2:
+                cells.append(q)
+            else:
+                cells.append([q[0], q[1], q[2], -1])
+                cells.append([q[0], q[2], q[3], -1])
+    cells = np.array(cells, np.int64)
+    x1, y1 = xs[-1], ys[-1]
+    tol = 1e-9 * h
+
+    def classify(mid, _dir):
+        pid = np.full(mid.shape[0], 0)
+        pid[np.abs(mid[:, 0] - x1) < tol] = 1
+        pid[np.abs(mid[:, 1]) < tol] = 2
+        pid[np.abs(mid[:, 1] - y1) < tol] = 3
+        return pid
+
+    return extrude_layers_3d(nodes, cells, nz, h, classify,
+                             (("inlet", "patch"), ("outlet", "patch"), ("Inflow", "patch"), ("inflow", "patch")))
+
+
 def box_3d(n: int, h: float = 1.0, nx=None, ny=None, nz=None) -> PolyMesh:
     """Uniform hex box with spacing h, patches inlet (x-) / outlet (x+) /
     Inflow (y-) / inflow (y+) / Outflow (z-) / outflow (z+)."""

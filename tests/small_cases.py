@@ -59,6 +59,24 @@ def poly(nx=9, ny=8, solver=0, mu=0.0, minmod=0, mach=0.3, **kw):
     return with_params(c, **kw) if kw else c
 
 
+def prism(nx=6, ny=4, nz=3, solver=0, mu=0.0, minmod=0, mach=0.3, **kw):
+    """Layers of hexes and triangular prisms (3D cells of 6 and 5 faces: the
+    face slots' placeholders in 3D) in a uniform flow along x, seeded
+    perturbation added."""
+    mesh = meshgen.prism_hex_3d(nx, ny, nz, h=1.0)
+    prm = cases.base_params(dim=3, solver=solver, mu=mu, minmod=minmod)
+    c_inf = cases.sound_speed(prm)
+    prm["U_inf"] = (mach * c_inf, 0.0, 0.0)
+    prm["mach"] = mach
+    prm["dt"] = 0.2 * 0.5 / ((1.0 + mach) * c_inf)
+    U = np.zeros((mesh.n_cells, 3))
+    U[:, 0] = prm["U_inf"][0]
+    c = cases.Case(f"prism{nx}x{ny}x{nz}", mesh, prm, np.full(mesh.n_cells, prm["p_inf"]),
+                   np.full(mesh.n_cells, prm["T_inf"]), U)
+    c = seeded_perturbation(c, amp=1e-2, seed=13)
+    return with_params(c, **kw) if kw else c
+
+
 def seeded_perturbation(case, amp=1e-3, seed=7):
     """Add a seeded velocity/pressure perturbation so every face carries a
     non-trivial flux (used on the otherwise uniform boxes)."""
@@ -93,6 +111,9 @@ PARITY_CASES = {
     # 2D cells of 5 and 6 faces (gather entries past the fourth)
     "poly_M1_euler": lambda: poly(solver=0),
     "poly_M2AUSM_visc_adjust": lambda: poly(solver=2, minmod=1, mu=0.02, adjust_dt=1, max_co=0.3),
+    # 3D cells of 5 faces (prisms) beside hexes
+    "prism_M1_euler": lambda: prism(solver=0),
+    "prism_M2_visc_adjust": lambda: prism(solver=1, mu=0.02, adjust_dt=1, max_co=0.3),
     # simulationType LES: calc_VIS_Smagorinsky
     "vortex_M2_les": lambda: vortex(24, solver=1, mu=0.05, laminar=0),
     "box_M2AUSM_les": lambda: with_params(seeded_perturbation(box(6, solver=2, mu=1e-3)), minmod=1, laminar=0),

@@ -28,6 +28,8 @@ CASES = {
     "vortex_rk0": lambda: sc.vortex(8, solver=0, rk_order=0),
     # 2D cells of 5 and 6 faces
     "poly_M2AUSM_visc_adjust": lambda: sc.PARITY_CASES["poly_M2AUSM_visc_adjust"](),
+    # 3D cells of 5 faces
+    "prism_M2_visc_adjust": lambda: sc.PARITY_CASES["prism_M2_visc_adjust"](),
 }
 
 

@@ -6,7 +6,7 @@ step must be identical bit for bit, since every face flux is the same
 function of the same inputs and every cell adds them in the same order.
 Covers 2D and 3D, M1 and M2, RK orders 1-5, the residual stage, the
 adjusted dt, wall / inlet / outlet ghosts, an unset patch whose ghosts
-never change, and 2D cells of 5 and 6 faces.
+never change, 2D cells of 5 and 6 faces and 3D cells of 5.
 """
 import copy
 
@@ -51,6 +51,9 @@ CASES = {
     # 2D cells of 5 and 6 faces: gather entries past the face slots
     "poly_M1": lambda: sc.poly(solver=0),
     "poly_M2_adjust": lambda: sc.poly(solver=1, adjust_dt=1, max_co=0.3),
+    # 3D cells of 5 faces: slot placeholders inside the 3D gathers
+    "prism_M1": lambda: sc.prism(solver=0),
+    "prism_M2_adjust": lambda: sc.prism(solver=1, adjust_dt=1, max_co=0.3),
 }
 
 

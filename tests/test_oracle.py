@@ -313,6 +313,60 @@ def test_polygon_strips_geometry_and_invariants(O):
         assert abs(t1[-1] - t0[-1]) <= 1e-13 * t0[-1]
 
 
+def test_prism_hex_layers_geometry_and_invariants(O):
+    """Layers of hexes and triangular prisms (3D cells of 6 and 5 faces):
+    volumes, centres and face closure in closed form, then freestream
+    preservation and, with every patch a wall, conservation of mass and
+    energy for each scheme."""
+    nx, ny, nz, h = 6, 4, 3, 1.0
+    m = meshgen.prism_hex_3d(nx, ny, nz, h=h)
+    case = sc.prism(nx, ny, nz, solver=0)
+    o = O.Oracle(m, case.params, np.full(m.n_cells, 1e5), np.full(m.n_cells, 300.0), np.zeros((m.n_cells, 3)))
+    vol = o.field(O.F_VOLUME)[0]
+    cc = o.field(O.F_CENTRE)
+    nface = o.slots()[0]
+    S = o.slots()[3]
+    n2 = ny * (nx // 2) + ny * (nx - nx // 2) * 2
+    c = 0
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                if i < nx // 2:
+                    shapes = [((i + 0.5) * h, (j + 0.5) * h, h ** 3, 6)]
+                else:
+                    # the two triangles of the split quad, centroids at 1/3
+                    shapes = [((i + 2 / 3) * h, (j + 1 / 3) * h, h ** 3 / 2, 5),
+                              ((i + 1 / 3) * h, (j + 2 / 3) * h, h ** 3 / 2, 5)]
+                for x, y, v, nf in shapes:
+                    assert vol[c] == pytest.approx(v, rel=1e-13)
+                    assert cc[0][c] == pytest.approx(x, abs=1e-12) and cc[1][c] == pytest.approx(y, abs=1e-12)
+                    assert cc[2][c] == pytest.approx((k + 0.5) * h, abs=1e-12)
+                    assert nface[c] == nf, (c, i, j, k)
+                    c += 1
+    assert c == nz * n2 == m.n_cells
+    assert np.abs(S.sum(axis=1)).max() < 1e-12
+    for solver in (0, 1, 2):
+        fs = sc.prism(nx, ny, nz, solver=solver, minmod=1, mu=0.01)
+        U = np.zeros((fs.n_cells, 3))
+        U[:, 0] = fs.params["U_inf"][0]
+        of = O.Oracle(fs.mesh, fs.params, np.full(fs.n_cells, fs.params["p_inf"]),
+                      np.full(fs.n_cells, fs.params["T_inf"]), U)
+        q0 = of.q()
+        of.run(3)
+        dq = np.abs(of.q() - q0)
+        assert dq[0].max() < 1e-13 and dq[-1].max() < 1e-12 and dq[1:-1].max() < 1e-13
+        wc = sc.prism(nx, ny, nz, solver=solver, minmod=1, mu=2e-3)
+        wm = wc.mesh
+        wm.patches = [meshgen.Patch(p.name, "wall", p.start, p.size) for p in wm.patches]
+        ow = O.Oracle(wm, wc.params, wc.p, wc.T, wc.U)
+        vw = ow.field(O.F_VOLUME)[0]
+        t0 = (ow.q() * vw).sum(axis=1)
+        ow.run(10)
+        t1 = (ow.q() * vw).sum(axis=1)
+        assert abs(t1[0] - t0[0]) <= 1e-13 * t0[0]
+        assert abs(t1[-1] - t0[-1]) <= 1e-13 * t0[-1]
+
+
 def test_closed_box_conserves_mass_and_energy(O):
     """All-wall box: wall ghosts mirror the velocity and keep rho, rhoE, so
     no mass or energy crosses the walls; the owner/neighbour scatter is
