@@ -51,11 +51,11 @@ struct KArgs32 {
     const double *vol_d;  // FP64 cell volumes (conservation integrals)
     double *d_cfl, *d_dtmin;  // this solver's FP64 CFL / dt-min scalars
     int nblk_cells;           // blocks of 256 over the cells (reduction partials)
-    // per-cell face slots of the one-launch stage and the gradient kernel
-    // (the first GatherUnroll gather entries of each cell, slot j of cell c
-    // at [j][c]): the face's
-    // other cell (~other where the cell is the face's N), its area vector
-    // ([j][k][c], face orientation) and weight; nullptr until built
+    // per-cell face slots of the one-launch stage and the 3D gradient kernel
+    // (the first GatherUnroll gather entries of each cell, stored as
+    // slot_at places them): the face's other cell (~other where the cell is
+    // the face's N; the cell itself past its last entry), its area vector
+    // (face orientation) and weight; nullptr until built
     const int32_t *sl_nb;
     const float *sl_S, *sl_w;
     float gamma, gm1, gog, R_inv, mu, cp, pr_inv;
@@ -1202,8 +1202,9 @@ __global__ void __launch_bounds__(256) cell_update32(KArgs32 a, float A, float B
 // the face kernel's own function, from the stage's input state a.q
 // (inviscid M1 / M2: no gradients), with the gathering cell's state and
 // primitives loaded and formed once per cell.  The face data come from the
-// cell's slots (a.sl_*: streamed, one record per gather entry) for its first
-// GatherUnroll entries and from the face arrays beyond them.  Each interior
+// cell's slots (a.sl_*: streamed, one per gather entry) for its first
+// GatherUnroll entries (all of a 3D cell's) and from the face arrays beyond
+// them.  Each interior
 // face is computed by both of its cells, to the same bits as the face
 // kernel's (the same function of the same values).
 template <int D, int SOLVER>
