@@ -302,7 +302,10 @@ int lfm_post_forces_read(lfm_solver *s, int32_t max_rows, double *rows, int32_t 
 typedef struct lfm_batch lfm_batch;
 int lfm_batch_create(lfm_batch **out);
 int lfm_batch_destroy(lfm_batch *b);
-/* Replace the member set (may be empty) and capture the step graph. */
+/* Replace the member set (may be empty) and capture the step graph.
+ * Inviscid M1 / M2 members get the one-launch stages' second state buffer
+ * and face slots here if they lack them (their lfm_info.device_bytes grows);
+ * where that memory cannot be had the batch runs separate launches. */
 int lfm_batch_set(lfm_batch *b, lfm_solver *const *solvers, int n);
 /* Queue n steps of every member on the batch's stream. */
 int lfm_batch_run(lfm_batch *b, int nsteps);

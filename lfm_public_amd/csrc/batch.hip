@@ -25,6 +25,10 @@ struct Batch {
     int nb_cells = 0, nb_faces = 0;
     int D = 2, solver = 0, minmod = 0, nstages = 0;
     bool visc = true;
+    // inviscid M1 / M2 members with their second buffers and face slots:
+    // the stages after the first `separate` ones are one launch each
+    bool fused = false;
+    int separate = 0;
     double A[5]{}, B[5]{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -38,10 +42,27 @@ struct Batch {
     }
 };
 
+template <int D, int SOLVER>
+static void enqueue_fused(Batch *b, int rk, int flip) {
+    const dim3 blk(256);
+    if (rk == b->nstages - 1)
+        hipLaunchKernelGGL((stage_fused32_b<D, SOLVER, 1>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
+                           b->map_cells, (float)b->A[rk], (float)b->B[rk], rk == 0 ? 1 : 0, flip);
+    else
+        hipLaunchKernelGGL((stage_fused32_b<D, SOLVER, -1>), dim3(b->nb_cells), blk, 0, b->stream, b->d_args,
+                           b->map_cells, (float)b->A[rk], (float)b->B[rk], rk == 0 ? 1 : 0, flip);
+}
+
 template <int D, bool V>
 static void enqueue_step(Batch *b) {
     const dim3 blk(256);
     for (int rk = 0; rk < b->nstages; rk++) {
+        if (b->fused && rk >= b->separate) {
+            // an even number of one-launch stages: each step ends in a.q
+            const int flip = (rk - b->separate) & 1;
+            b->solver == 0 ? enqueue_fused<D, 0>(b, rk, flip) : enqueue_fused<D, 1>(b, rk, flip);
+            continue;
+        }
         // ghosts are refreshed by the update kernel (members join with fresh ghosts);
         // without viscosity solvers 0/1 consume no gradient or tensor
         if (V || b->solver == 2) {
@@ -74,6 +95,11 @@ static void enqueue_step(Batch *b) {
 static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n);
 
 static int batch_set(Batch *b, lfm_solver *const *hs, int n) {
+    // the one-launch stages' buffers of eligible members, allocated (under
+    // the set-up mutex, which ensure_alt32 takes itself) before the batch
+    // takes it; a member without them keeps the batch on separate launches
+    for (int i = 0; i < n; i++)
+        if (hs && hs[i] && hs[i]->impl && hs[i]->impl->fp32) (void)hs[i]->impl->ensure_alt32();
     std::lock_guard<std::mutex> lock(device_setup_mutex());
     return batch_set_locked(b, hs, n);
 }
@@ -89,7 +115,9 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
         b->graph = nullptr;
     }
     b->members.clear();
+    b->fused = false;
     if (n == 0) return LFM_OK;
+    bool fused = true;
     std::vector<KArgs32> args;
     std::vector<int32_t> req_c, req_f, first_c, first_f;
     for (int i = 0; i < n; i++) {
@@ -123,6 +151,8 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
         HIP_TRY(hipStreamSynchronize(s->stream));
         b->members.push_back(s);
         args.push_back(*s->k32);
+        args.back().q_alt = s->q_alt32;
+        fused = fused && s->fused_ok && s->q_alt32 && s->k32->sl_nb && !s->post_forces;
         auto add = [&](std::vector<int32_t> &req, std::vector<int32_t> &first, long count) {
             first.push_back((int32_t)req.size());
             const long nb = (count + 255) / 256;
@@ -133,6 +163,8 @@ static int batch_set_locked(Batch *b, lfm_solver *const *hs, int n) {
     }
     b->nb_cells = (int)req_c.size();
     b->nb_faces = (int)req_f.size();
+    b->fused = fused && b->solver < 2 && !b->visc;
+    b->separate = b->nstages % 2;  // leaves an even number of one-launch stages
     const size_t nmap = req_c.size() + req_f.size() + 2 * (size_t)n;
     if (nmap > b->maps_cap) {
         if (b->d_maps) HIP_TRY(hipFree(b->d_maps));

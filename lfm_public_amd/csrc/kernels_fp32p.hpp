@@ -58,6 +58,9 @@ struct KArgs32 {
     // (face orientation) and weight; nullptr until built
     const int32_t *sl_nb;
     const float *sl_S, *sl_w;
+    // the second state buffer, in a batch's copy of a member's arguments
+    // (the solver's own copy leaves it null)
+    float *q_alt;
     float gamma, gm1, gog, R_inv, mu, cp, pr_inv;
     float rho_inf, p_inf, T_inf, Rpsi_inf, E_inf, H_inf, G_inf, c_inf;
     float uinf[3], minf[3];
@@ -1402,6 +1405,21 @@ __global__ void __launch_bounds__(256) cell_update32_b(const KArgs32 *args, Batc
     const KArgs32 &a = args[r];
     cell_update32_body<D, false, CFL>(a, FluxGlobal32<D>{a}, a.q, A, B, zero_dq, lb * 256 + threadIdx.x, lb,
                                       CFL >= 0 ? *a.dt : 0.0);
+}
+// one-launch stage over a batch (inviscid M1 / M2), each member from one
+// of its state buffers into the other: flip 0 from a.q into a.q_alt, flip 1
+// back
+template <int D, int SOLVER, int CFL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 3 ? (CFL < 0 ? 7 : 6) : 1)))
+stage_fused32_b(const KArgs32 *args, BatchMap m, float A, float B, int zero_dq, int flip) {
+    __shared__ float keep[CFL >= 0 ? GatherUnroll<D>::value * D * 256 : 1];
+    int r, lb;
+    batch_slot(m, r, lb);
+    KArgs32 a = args[r];
+    float *qout = flip ? a.q : a.q_alt;
+    if (flip) a.q = a.q_alt;
+    cell_update32_body<D, false, CFL>(a, FluxSlots32<D, SOLVER>{a, CFL >= 0 ? keep : nullptr}, qout, A, B, zero_dq,
+                                      lb * 256 + threadIdx.x, lb, CFL >= 0 ? *a.dt : 0.0);
 }
 // one block per solver: max of its partials into its CFL scalar
 static __global__ void __launch_bounds__(256) cfl_final32_b(const KArgs32 *args) {

@@ -56,9 +56,13 @@ class Request:
         return self.n * self.n
 
 
-def est_device_bytes(n_cells: int) -> int:
-    """Resident FP32-perturbation footprint of a 2D solve (measured ~330 B/cell)."""
-    return int(360 * n_cells + (1 << 20))
+def est_device_bytes(n_cells: int, kind: Optional[str] = None) -> int:
+    """Resident FP32-perturbation footprint of a 2D solve (measured ~330 B/cell);
+    a vortex request (inviscid M1) also holds, once batched, the one-launch
+    stages' second state buffer and face slots (~80 B/cell).  Without a kind,
+    the larger."""
+    per = 360 if kind == "cylinder" else 440
+    return int(per * n_cells + (1 << 20))
 
 
 _mesh_cache: Dict[Tuple[str, int], object] = {}
@@ -160,7 +164,7 @@ class Server:
             # admission: FIFO under the HBM budget (head-of-line waits)
             while waiting:
                 r = waiting[0]
-                est = est_device_bytes(r.n_cells)
+                est = est_device_bytes(r.n_cells, r.kind)
                 if reserved + est > self.budget and (reserved > 0 or building):
                     break
                 waiting.pop(0)
@@ -170,7 +174,10 @@ class Server:
             for fut in [f for f in building if f.done()]:
                 r, est, t_adm = building.pop(fut)
                 s = fut.result()
-                nbytes = s.info().device_bytes
+                # the estimate stays reserved until the batch has allocated
+                # what it needs (the refresh below), unless the solve already
+                # holds more
+                nbytes = max(s.info().device_bytes, est)
                 reserved += nbytes - est
                 active.setdefault(r.cls, []).append(Resident(r, s, r.nsteps, nbytes, t_adm, self.now() - t0))
                 dirty.add(r.cls)
@@ -182,6 +189,12 @@ class Server:
                     b = self.batches[cls] = capi.Batch()
                 b.set([m.solver for m in members])
                 max_members = max(max_members, len(members))
+                # joining a batch may allocate the one-launch stages' buffers
+                for m in members:
+                    nb = m.solver.info().device_bytes
+                    reserved += nb - m.nbytes
+                    m.nbytes = nb
+            peak_resident = max(peak_resident, reserved)
             dirty.clear()
             residents = [m for ms in active.values() for m in ms]
             if not residents:
