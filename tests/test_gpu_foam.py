@@ -62,7 +62,7 @@ def _read_scalar(path):
 
 
 def test_solve_driver_post_processing(tmp_path):
-    """lfm/post haveAverage + haveForces through the driver: pAvg / pRMS in
+    """lfm/post haveAverage + haveForces through solve.py (lfm_solve): pAvg / pRMS in
     the time directories and output/postProcess_Forces_<k> per wall patch,
     against the oracle stepping the same case."""
     import oracle as O
