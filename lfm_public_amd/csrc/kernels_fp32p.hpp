@@ -904,6 +904,7 @@ struct FaceList32 {
     const KArgs32 &a;
     static constexpr int kUnroll = UpdateUnroll32<D>::value;
     static constexpr bool kAllSlots = false;
+    static constexpr bool kSlotsFirst = false;
     __device__ __forceinline__ static bool valid(int key, int c) { return (void)key, (void)c, true; }
     __device__ __forceinline__ void begin_cell(int c) const { (void)c; }
     __device__ __forceinline__ int key(int e, int j, int c) const {
@@ -991,7 +992,9 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
     fa.begin_cell(c);
 #pragma unroll
     for (int j0 = 0; j0 < kMaxFaces; j0 += U) {
-        if (j0 < ne) {
+        // every cell has a first group: where its loads do not address
+        // through the gather list, they need not wait for ne
+        if ((j0 == 0 && FA::kSlotsFirst) || j0 < ne) {
             int key[U];
             float fr[U][D + 2];
 #pragma unroll
@@ -1226,6 +1229,10 @@ struct FluxSlots32 {
     static constexpr int kUnroll = D == 3 ? 2 : U;
     // a 3D cell's entries all fit the slots (kMaxFaces of them)
     static constexpr bool kAllSlots = U >= kMaxFaces;
+    // the first kUnroll entries are slots, addressed by the cell alone, so
+    // their loads can issue before the entry count arrives (2D: 0.0271 ->
+    // 0.0263 ms per 65 k-cell step; the 3D box showed no gain)
+    static constexpr bool kSlotsFirst = D == 2;
     // a slot past the cell's last entry holds the cell itself
     __device__ __forceinline__ static bool valid(int key, int c) { return key != c; }
     mutable float qc[D + 2];
