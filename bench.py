@@ -90,7 +90,9 @@ def launch_plan(prm, nstages):
     (but a residual stage 0) as one stage_fused32 launch; other paths run
     grad_vis (when it has work), face_flux and cell_update per stage.  The
     last stage's launch also forms the CFL/dt block extrema, except on the
-    FP64 3D path, where they keep a launch of their own."""
+    FP64 3D path, where they keep a launch of their own.  Their final
+    reduction is one single-block launch, or, inside a run that steps
+    again, one extra block of the next step's first FP32 launch."""
     D, S = prm["dim"], prm["solver"]
     fp32 = prm["precision"] == 32
     visc = prm["mu"] != 0.0 or not prm.get("laminar", 1)

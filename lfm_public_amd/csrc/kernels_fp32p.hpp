@@ -1189,10 +1189,21 @@ template <int D>
 __global__ void __launch_bounds__(256) bc_fill32(KArgs32 a) {
     bc_fill32_body<D>(a, LFM_TID(blockIdx.x));
 }
+// fin_nblk > 0: one block more, the last, reduces the previous step's
+// fin_nblk per-block CFL maxima in a.partial into *a.d_cfl (as the one-launch
+// stage does)
 template <int D, int GRAD, bool VISC, bool RG, bool SL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 3 && SL && !RG ? 5 : 1)))
-grad_vis32(KArgs32 a) {
-    grad_vis32_body<D, GRAD, VISC, RG, SL>(a, LFM_TID(blockIdx.x));
+grad_vis32(KArgs32 a, int fin_nblk) {
+    int nb = (int)gridDim.x;
+    if (fin_nblk > 0) {
+        nb -= 1;
+        if ((int)blockIdx.x == nb) {
+            cfl_dt_final_block<1, 256>(a.partial, fin_nblk, a.d_cfl, nullptr);
+            return;
+        }
+    }
+    grad_vis32_body<D, GRAD, VISC, RG, SL>(a, xcd_block((int)blockIdx.x, nb) * (int)blockDim.x + (int)threadIdx.x);
 }
 // 3D M1 / M2 faces: 5 waves per SIMD fit in 96 VGPRs without spills (97
 // would allow 4): O-grid face kernel 1.107 -> 1.033 ms per step.  Forcing
@@ -1337,13 +1348,30 @@ __global__ void __launch_bounds__(256) build_slots32(KArgs32 a, int32_t *nb, flo
 // 3D: at least 7 waves per SIMD (6 for the last stage, whose CFL/dt sums
 // keep more live; both compile to 8), with two faces' loads in flight per
 // wave (kUnroll).
+// fin_nblk > 0 (a stage without CFL/dt sums): the grid has one block more,
+// the last, which reduces the fin_nblk per-block CFL maxima that an earlier
+// launch left in a.partial into *a.d_cfl (the final reduction of the
+// previous step, whose result no stage of this step reads).
 template <int D, int SOLVER, int CFL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D == 3 ? (CFL < 0 ? 7 : 6) : 1)))
-stage_fused32(KArgs32 a, float *qout, float A, float B, int zero_dq, double cfl_param) {
+stage_fused32(KArgs32 a, float *qout, float A, float B, int zero_dq, double cfl_param, int fin_nblk) {
     // the last stage keeps its cells' area vectors in LDS for the CFL/dt sums
     __shared__ float keep[CFL >= 0 ? GatherUnroll<D>::value * D * 256 : 1];
+    int nb = (int)gridDim.x;
+    if constexpr (CFL < 0) {
+        if (fin_nblk > 0) {
+            nb -= 1;
+            if ((int)blockIdx.x == nb) {
+                cfl_dt_final_block<1, 256>(a.partial, fin_nblk, a.d_cfl, nullptr);
+                return;
+            }
+        }
+    } else {
+        (void)fin_nblk;
+    }
+    const int blk = xcd_block((int)blockIdx.x, nb);
     cell_update32_body<D, false, CFL>(a, FluxSlots32<D, SOLVER>{a, CFL >= 0 ? keep : nullptr}, qout, A, B, zero_dq,
-                                      LFM_TID(blockIdx.x), LFM_BLK(), cfl_param);
+                                      blk * (int)blockDim.x + (int)threadIdx.x, blk, cfl_param);
 }
 template <int D, int MODE>
 __global__ void __launch_bounds__(256) cfl_dt_cells32(KArgs32 a, double param) {
@@ -1452,7 +1480,7 @@ static __global__ void __launch_bounds__(256) cfl_final32_b(const KArgs32 *args)
 template <int D>
 void launch_bc32(Solver *s);
 template <int D>
-void launch_gradvis32(Solver *s, bool full);
+void launch_gradvis32(Solver *s, bool full, bool fin = false);
 template <int D>
 void launch_flux32(Solver *s);
 template <int D>
@@ -1464,6 +1492,6 @@ template <int D>
 void launch_build_slots32(Solver *s, int32_t *nb, float *S, float *w);
 template <int D>
 void launch_stage_fused32(Solver *s, double A, double B, bool zero_dq, const float *qin, float *qout,
-                          const Solver::CflJob *cfl);
+                          const Solver::CflJob *cfl, bool fin);
 
 }  // namespace lfm

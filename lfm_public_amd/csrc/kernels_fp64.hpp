@@ -1088,40 +1088,46 @@ __global__ void __launch_bounds__(256) cfl_dt_cells(KArgs a, double param) {
     block_extremum<MODE>(v, a.partial + LFM_BLK());
 }
 
-// Final min / max over the per-block values: one block of kFinalBlock
-// threads, each with 16 loads in flight per round, so the latency of a few
-// rounds (65 536 partials on 16.8 M cells: 4) bounds it rather than a long
-// dependent chain.  Min / max are exact in any order.
-constexpr int kFinalBlock = 1024;
-template <int MODE>
-static __global__ void __launch_bounds__(kFinalBlock) cfl_dt_final(const double *partial, int nblk, double *out,
-                                                                  double *dt_dst) {
+// Final min / max over the per-block values by one block of NT threads,
+// each with 16 loads in flight per round, so the latency of a few rounds
+// (65 536 partials on 16.8 M cells: 4 with 1024 threads) bounds it rather
+// than a long dependent chain.  Min / max are exact in any order.
+template <int MODE, int NT>
+__device__ __forceinline__ void cfl_dt_final_block(const double *partial, int nblk, double *out, double *dt_dst) {
     auto pick = [](double x, double o) {
         if (MODE == 0) return x > o ? o : x;
         return x < o ? o : x;
     };
     constexpr int kLoads = 16;
     double v = MODE == 0 ? 100.0 : 0.0;
-    for (int i0 = 0; i0 < nblk; i0 += kLoads * kFinalBlock) {
+    for (int i0 = 0; i0 < nblk; i0 += kLoads * NT) {
         double o[kLoads];
 #pragma unroll
         for (int j = 0; j < kLoads; j++) {
-            const int i = i0 + j * kFinalBlock + (int)threadIdx.x;
+            const int i = i0 + j * NT + (int)threadIdx.x;
             o[j] = i < nblk ? partial[i] : v;
         }
 #pragma unroll
         for (int j = 0; j < kLoads; j++) v = pick(v, o[j]);
     }
     for (int off = 32; off > 0; off >>= 1) v = pick(v, __shfl_down(v, off, 64));
-    __shared__ double w[kFinalBlock / 64];
+    __shared__ double w[NT / 64];
     if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = w[0];
-        for (int i = 1; i < kFinalBlock / 64; i++) r = pick(r, w[i]);
+        for (int i = 1; i < NT / 64; i++) r = pick(r, w[i]);
         *out = r;
         if (dt_dst) *dt_dst = r;
     }
+}
+
+// the final reduction as a launch of its own: one block of kFinalBlock
+constexpr int kFinalBlock = 1024;
+template <int MODE>
+static __global__ void __launch_bounds__(kFinalBlock) cfl_dt_final(const double *partial, int nblk, double *out,
+                                                                  double *dt_dst) {
+    cfl_dt_final_block<MODE, kFinalBlock>(partial, nblk, out, dt_dst);
 }
 
 // conservation integrals sum_c V_c q_c, per-block partials (FP64)

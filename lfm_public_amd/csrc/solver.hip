@@ -271,9 +271,9 @@ void Solver::bc() {
     }
 }
 
-void Solver::gradvis(bool full) {
+void Solver::gradvis(bool full, bool fin) {
     if (fp32) {
-        LFM_DISPATCH_D(this, launch_gradvis32<2>(this, full), launch_gradvis32<3>(this, full));
+        LFM_DISPATCH_D(this, launch_gradvis32<2>(this, full, fin), launch_gradvis32<3>(this, full, fin));
     } else {
         LFM_DISPATCH_D(this, launch_gradvis64<2>(this, full), launch_gradvis64<3>(this, full));
     }
@@ -337,8 +337,11 @@ void Solver::stage_a(int rk) {
         prof_mark(0, false);
         ghosts_stale = false;
     }
+    // a carried CFL reduction is taken by the step's first gradient launch
+    const bool fin = cfl_pending && carry_target() == 2;
+    if (fin) cfl_pending = false;
     prof_mark(1, true);
-    gradvis(false);
+    gradvis(false, fin);
     prof_mark(1, false);
 }
 
@@ -364,9 +367,7 @@ void Solver::stage_b(int rk, bool fuse_cfl, bool own_dt) {
         prof_mark(5, true);
         update(A[rk], B[rk], rk == 0, false, false, &job);
         prof_mark(5, false);
-        prof_mark(4, true);
-        cfl_final(job);
-        prof_mark(4, false);
+        final_or_carry(job);
         cfl_fused = true;
     } else {
         prof_mark(3, true);
@@ -460,24 +461,57 @@ void Solver::stage_fused(int rk, bool from_alt) {
     if (rk == nstages - 1) {
         const CflJob job = cfl_job(true);
         prof_mark(5, true);
-        LFM_DISPATCH_D(this, launch_stage_fused32<2>(this, A[rk], B[rk], rk == 0, qin, qout, &job),
-                       launch_stage_fused32<3>(this, A[rk], B[rk], rk == 0, qin, qout, &job));
+        LFM_DISPATCH_D(this, launch_stage_fused32<2>(this, A[rk], B[rk], rk == 0, qin, qout, &job, false),
+                       launch_stage_fused32<3>(this, A[rk], B[rk], rk == 0, qin, qout, &job, false));
         prof_mark(5, false);
-        prof_mark(4, true);
-        cfl_final(job);
-        prof_mark(4, false);
+        final_or_carry(job);
         cfl_fused = true;
     } else {
+        // a carried reduction is taken by the step's first stage
+        const bool fin = cfl_pending && carry_target() == 1;
+        if (fin) cfl_pending = false;
         prof_mark(3, true);
-        LFM_DISPATCH_D(this, launch_stage_fused32<2>(this, A[rk], B[rk], rk == 0, qin, qout, nullptr),
-                       launch_stage_fused32<3>(this, A[rk], B[rk], rk == 0, qin, qout, nullptr));
+        LFM_DISPATCH_D(this, launch_stage_fused32<2>(this, A[rk], B[rk], rk == 0, qin, qout, nullptr, fin),
+                       launch_stage_fused32<3>(this, A[rk], B[rk], rk == 0, qin, qout, nullptr, fin));
         prof_mark(3, false);
     }
+}
+
+// the launch of a step that can take the previous step's carried CFL
+// reduction before anything writes the reduction scratch: 1, the first
+// one-launch stage (not a residual stage, and not the only stage, which
+// forms the CFL itself); 2, the first FP32 gradient launch (each stage of a
+// viscous or M2AUSM solver has one, before its update); 0, none
+int Solver::carry_target() const {
+    if (!L.halo.empty()) return 0;
+    if (fused_ok) return q_alt32 && fused_from() == 0 && nstages > 1 ? 1 : 0;
+    return fp32 && (visc || prm.solver == 2) ? 2 : 0;
+}
+
+// the final reduction of a CFL job the last stage formed: carried into the
+// next step (compute_cfl inside a run that steps again) or launched now
+void Solver::final_or_carry(const CflJob &job) {
+    if (defer_cfl && job.mode == 1 && job.out == d_cfl && carry_target() != 0) {
+        cfl_pending = true;
+        return;
+    }
+    prof_mark(4, true);
+    cfl_final(job);
+    prof_mark(4, false);
+}
+
+void Solver::flush_cfl() {
+    if (!cfl_pending) return;
+    cfl_pending = false;
+    prof_mark(4, true);
+    cfl_final(CflJob{1, dt, d_cfl, nullptr});
+    prof_mark(4, false);
 }
 
 // one full time step (Mesh::solve loop body), no host synchronisation
 void Solver::step() {
     const int first = fused_ok && ensure_alt32() ? fused_from() : nstages;
+    if (carry_target() == 0) flush_cfl();
     for (int rk = 0; rk < first; rk++) {
         stage_a(rk);
         stage_b(rk, true, true);
@@ -684,12 +718,14 @@ int lfm_run(lfm_solver *h, int nsteps) {
         return fail(LFM_ESTATE, "a partition with processor patches steps through lfm_part_group_run or the "
                                 "per-stage calls with halo exchange");
     for (int i = 0; i < nsteps; i++) {
+        s->defer_cfl = i + 1 < nsteps && !s->post;
         s->step();
         if (s->post) {
             const int rc = s->post_step();
             if (rc) return rc;
         }
     }
+    s->flush_cfl();
     HIP_TRY(hipGetLastError());
     return LFM_OK;
 }
@@ -707,7 +743,11 @@ int lfm_time_steps(lfm_solver *h, int nsteps, double *ms_out) {
         return fail(LFM_ESTATE, "a partition with processor patches steps through lfm_part_group_run or the "
                                 "per-stage calls with halo exchange");
     HIP_TRY(hipEventRecord(s->ev0, s->stream));
-    for (int i = 0; i < nsteps; i++) s->step();
+    for (int i = 0; i < nsteps; i++) {
+        s->defer_cfl = i + 1 < nsteps;
+        s->step();
+    }
+    s->flush_cfl();
     HIP_TRY(hipEventRecord(s->ev1, s->stream));
     HIP_TRY(hipEventSynchronize(s->ev1));
     float ms = 0.f;
@@ -730,7 +770,11 @@ int lfm_profile_steps(lfm_solver *h, int nsteps, double *ms_per_group, double *m
     s->prof_n = 0;
     s->prof = true;
     HIP_TRY(hipEventRecord(s->ev0, s->stream));
-    for (int i = 0; i < nsteps; i++) s->step();
+    for (int i = 0; i < nsteps; i++) {
+        s->defer_cfl = i + 1 < nsteps;
+        s->step();
+    }
+    s->flush_cfl();
     HIP_TRY(hipEventRecord(s->ev1, s->stream));
     s->prof = false;
     HIP_TRY(hipEventSynchronize(s->ev1));

@@ -173,31 +173,35 @@ void launch_bc32(Solver *s) {
 }
 
 template <int D, bool V, bool SL>
-static void launch_gradvis32_v(Solver *s) {
+static void launch_gradvis32_v(Solver *s, bool fin) {
     const int nb = nblocks(s->L.nc);
+    const dim3 grid(nb + (fin ? 1 : 0));
+    const int fb = fin ? nb : 0;
     if (s->prm.solver == 2 && s->prm.minmod)
-        hipLaunchKernelGGL((grad_vis32<D, 2, V, true, SL>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+        hipLaunchKernelGGL((grad_vis32<D, 2, V, true, SL>), grid, dim3(kBlock), 0, s->stream, *s->k32, fb);
     else if (s->prm.solver == 2)
-        hipLaunchKernelGGL((grad_vis32<D, 0, V, true, SL>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+        hipLaunchKernelGGL((grad_vis32<D, 0, V, true, SL>), grid, dim3(kBlock), 0, s->stream, *s->k32, fb);
     else
-        hipLaunchKernelGGL((grad_vis32<D, 0, V, false, SL>), dim3(nb), dim3(kBlock), 0, s->stream, *s->k32);
+        hipLaunchKernelGGL((grad_vis32<D, 0, V, false, SL>), grid, dim3(kBlock), 0, s->stream, *s->k32, fb);
 }
 
 // 3D gradients read the face slots (same bits as the gather list; 4.2 M-cell
 // O-grid 0.98 -> 0.86 ms per step).  2D gradients keep the list: there the
 // slot streams cost more than the cached face records they replace (1 M-cell
-// annulus 0.114 -> 0.134 ms per step with slots).
+// annulus 0.114 -> 0.134 ms per step with slots).  fin: the launch also
+// reduces the previous step's CFL maxima in one extra block (the caller
+// knows the launch happens: full, viscous or M2AUSM).
 template <int D>
-void launch_gradvis32(Solver *s, bool full) {
+void launch_gradvis32(Solver *s, bool full, bool fin) {
     if (!full && !s->visc && s->prm.solver != 2) return;
     const bool V = full || s->visc;
     if constexpr (D == 3) {
         if (s->ensure_slots32()) {
-            V ? launch_gradvis32_v<D, true, true>(s) : launch_gradvis32_v<D, false, true>(s);
+            V ? launch_gradvis32_v<D, true, true>(s, fin) : launch_gradvis32_v<D, false, true>(s, fin);
             return;
         }
     }
-    V ? launch_gradvis32_v<D, true, false>(s) : launch_gradvis32_v<D, false, false>(s);
+    V ? launch_gradvis32_v<D, true, false>(s, fin) : launch_gradvis32_v<D, false, false>(s, fin);
 }
 
 template <int D, bool V, bool ST>
@@ -239,32 +243,34 @@ void launch_update32(Solver *s, double A, double B, bool zero_dq, bool res, bool
     }
 }
 
+// fin: one block more reduces the CFL maxima of the previous step's last
+// stage (left in a.partial) into a.d_cfl
 template <int D, int SOLVER>
 static void launch_fused32_v(Solver *s, float A, float B, int zero_dq, const KArgs32 &a, float *qout,
-                             const Solver::CflJob *cfl) {
+                             const Solver::CflJob *cfl, bool fin) {
     const int nb = nblocks(s->L.nc);
     if (!cfl) {
-        hipLaunchKernelGGL((stage_fused32<D, SOLVER, -1>), dim3(nb), dim3(kBlock), 0, s->stream, a, qout, A, B,
-                           zero_dq, 0.0);
+        hipLaunchKernelGGL((stage_fused32<D, SOLVER, -1>), dim3(nb + (fin ? 1 : 0)), dim3(kBlock), 0, s->stream, a,
+                           qout, A, B, zero_dq, 0.0, fin ? nb : 0);
         return;
     }
     if (cfl->mode == 0)
         hipLaunchKernelGGL((stage_fused32<D, SOLVER, 0>), dim3(nb), dim3(kBlock), 0, s->stream, a, qout, A, B,
-                           zero_dq, cfl->param);
+                           zero_dq, cfl->param, 0);
     else
         hipLaunchKernelGGL((stage_fused32<D, SOLVER, 1>), dim3(nb), dim3(kBlock), 0, s->stream, a, qout, A, B,
-                           zero_dq, cfl->param);
+                           zero_dq, cfl->param, 0);
 }
 
 template <int D>
 void launch_stage_fused32(Solver *s, double A, double B, bool zero_dq, const float *qin, float *qout,
-                          const Solver::CflJob *cfl) {
+                          const Solver::CflJob *cfl, bool fin) {
     KArgs32 a = *s->k32;
     a.q = const_cast<float *>(qin);
     if (s->prm.solver == 0)
-        launch_fused32_v<D, 0>(s, (float)A, (float)B, (int)zero_dq, a, qout, cfl);
+        launch_fused32_v<D, 0>(s, (float)A, (float)B, (int)zero_dq, a, qout, cfl, fin);
     else
-        launch_fused32_v<D, 1>(s, (float)A, (float)B, (int)zero_dq, a, qout, cfl);
+        launch_fused32_v<D, 1>(s, (float)A, (float)B, (int)zero_dq, a, qout, cfl, fin);
 }
 
 template <int D>
@@ -396,16 +402,16 @@ int nonfinite32(Solver *s, unsigned long long *d) {
 
 template void launch_bc32<2>(Solver *);
 template void launch_bc32<3>(Solver *);
-template void launch_gradvis32<2>(Solver *, bool);
-template void launch_gradvis32<3>(Solver *, bool);
+template void launch_gradvis32<2>(Solver *, bool, bool);
+template void launch_gradvis32<3>(Solver *, bool, bool);
 template void launch_flux32<2>(Solver *);
 template void launch_flux32<3>(Solver *);
 template void launch_update32<2>(Solver *, double, double, bool, bool, bool, const Solver::CflJob *);
 template void launch_update32<3>(Solver *, double, double, bool, bool, bool, const Solver::CflJob *);
 template void launch_stage_fused32<2>(Solver *, double, double, bool, const float *, float *,
-                                     const Solver::CflJob *);
+                                     const Solver::CflJob *, bool);
 template void launch_stage_fused32<3>(Solver *, double, double, bool, const float *, float *,
-                                     const Solver::CflJob *);
+                                     const Solver::CflJob *, bool);
 template void launch_build_slots32<2>(Solver *, int32_t *, float *, float *);
 template void launch_build_slots32<3>(Solver *, int32_t *, float *, float *);
 template void launch_cfl_dt32<2>(Solver *, int, double, double *, double *);

@@ -128,7 +128,7 @@ struct Solver {
 
     // stage pieces on the handle's stream
     void bc();
-    void gradvis(bool full);
+    void gradvis(bool full, bool fin = false);
     void flux();
     // cfl: the update also forms that reduction from the updated cells (it
     // reads only a cell's own state and its faces' areas); not with res
@@ -160,6 +160,19 @@ struct Solver {
     bool ensure_slots32();
     int fused_from() const;
     void stage_fused(int rk, bool from_alt);
+    // the compute_cfl final reduction carried into the next step: when
+    // defer_cfl is set (by a loop that steps again at once) and the next
+    // step begins with a launch that can take it (carry_target), the last
+    // stage leaves its per-block CFL maxima in the reduction scratch and
+    // that launch reduces them in one extra block (no stage reads the CFL
+    // at a fixed deltaT), which saves the single-block launch.  flush_cfl()
+    // launches a carried reduction on its own; the loops call it when they
+    // end.
+    bool defer_cfl = false;
+    bool cfl_pending = false;
+    int carry_target() const;
+    void final_or_carry(const CflJob &job);
+    void flush_cfl();
     void end_step(bool own_dt);
     void zero_state_for_step();
 

@@ -121,3 +121,48 @@ def test_batched_fused_stage_equals_lone_runs(rk_order):
         lone.close()
         members[i].close()
         assert np.array_equal(ql, qb), (rk_order, i, np.abs(ql - qb).max())
+
+
+# three-launch stages (viscous or M2AUSM): the first gradient launch takes it
+MULTI_STEP = dict(CASES)
+MULTI_STEP.update({
+    "vortex_M2AUSM_minmod": sc.PARITY_CASES["vortex_M2AUSM_minmod"],
+    "vortex_M2_visc_rk1": lambda: sc.vortex(16, solver=1, mu=0.05, rk_order=1),
+    "cylinder_M2AUSM_sponge_res": sc.PARITY_CASES["cylinder_M2AUSM_sponge"],
+    "box_M1_visc": sc.PARITY_CASES["box_M1_visc"],
+    "ogrid_M2AUSM_cyclic": sc.PARITY_CASES["ogrid_M2AUSM_cyclic"],
+    "poly_M2AUSM_visc_adjust": sc.PARITY_CASES["poly_M2AUSM_visc_adjust"],
+    "vortex_M2_les": sc.PARITY_CASES["vortex_M2_les"],
+})
+
+
+@pytest.mark.parametrize("name", ["vortex_M1_rk5", "vortex_M1_rk1", "vortex_M1_rk2_res", "vortex_M1_adjust",
+                                  "cylinder_M1_sponge", "box_M1", "prism_M1", "poly_M1", "vortex_M2AUSM_minmod",
+                                  "vortex_M2_visc_rk1", "cylinder_M2AUSM_sponge_res", "box_M1_visc",
+                                  "ogrid_M2AUSM_cyclic", "poly_M2AUSM_visc_adjust", "vortex_M2_les"])
+def test_multi_step_runs_equal_single_steps(name):
+    """Steps taken by one run(n) call (where each step's compute_cfl final
+    reduction is carried into the next step's first one-launch stage or
+    gradient launch as one extra block, and the last step's runs on its
+    own) end bit-identical, with the same CFL / deltaT, to the same steps
+    taken one run(1) call at a time.  Inviscid one-stage schemes, inviscid
+    residual stages and the adjusted deltaT do not carry it; the timed loop
+    (lfm_time_steps) carries it like run(n)."""
+    print(f"[fused] multi-step {name}", flush=True)
+    c = MULTI_STEP[name]()
+    many = capi.GpuSolver.from_case(c, precision=capi.PREC_FP32P)
+    one = capi.GpuSolver.from_case(c, precision=capi.PREC_FP32P)
+    timed = capi.GpuSolver.from_case(c, precision=capi.PREC_FP32P)
+    many.run(7)
+    for _ in range(7):
+        one.run(1)
+    timed.time_steps(7)
+    a, b, t = many.q(), one.q(), timed.q()
+    assert np.array_equal(a, b, equal_nan=True), np.nanmax(np.abs(a - b))
+    assert np.array_equal(t, b, equal_nan=True), np.nanmax(np.abs(t - b))
+    ia, ib, it = many.info(), one.info(), timed.info()
+    assert ia.cfl == ib.cfl == it.cfl and ia.dt == ib.dt == it.dt, (ia.cfl, ib.cfl, it.cfl, ia.dt, ib.dt)
+    if not c.params.get("adjust_dt"):
+        assert many.compute_cfl(ia.dt) == ia.cfl > 0
+    for g in (many, one, timed):
+        g.close()
