@@ -1,13 +1,13 @@
-"""Parity at the benchmark configurations' full sizes (needs an MI355X).
+"""Parity at the benchmark workloads' full sizes (needs an MI355X).
 
 FP64 against the oracle on the same inputs, at sizes the oracle steps in
-seconds: config 1 (256^2 isentropic vortex, 65,536 cells, M1, 40 RK5
-steps), config 2 (1,048,576-cell mixed quad/triangle annulus, M2AUSM +
-minmod, viscous, sponge, 2 steps) and config 3 (4,194,304-cell 3D cylinder
-O-grid, M2 + minmodExists, viscous, spanwise cyclic, 1 step).  Bar: relative
-L2 <= 1e-10 on p, U and T (the FP64 kernels are expected bit-identical).
+seconds: the 256^2 isentropic vortex (65,536 cells, M1, 40 RK5 steps), the
+1,048,576-cell mixed quad/triangle annulus (M2AUSM + minmod, viscous,
+sponge, 2 steps) and the 4,194,304-cell 3D cylinder O-grid (M2 +
+minmodExists, viscous, spanwise cyclic, 1 step).  Bar: relative L2 <= 1e-10
+on p, U and T (the FP64 kernels are expected bit-identical).
 
-FP32P against FP64 at config 2's full size after 200 steps.  Bar:
+FP32P against FP64 on the full-size annulus after 200 steps.  Bar:
 perturbation-normalised relative L2 <= 1e-5 on p, U and T.
 """
 import numpy as np
@@ -19,10 +19,10 @@ from lfm_public_amd import capi, cases
 pytestmark = pytest.mark.gpu
 
 
-def _config(name):
-    if name == "config1_vortex256":
+def _workload(name):
+    if name == "vortex256":
         return cases.isentropic_vortex(nx=256), 40
-    if name == "config2_annulus1M":
+    if name == "annulus1M":
         n = 1024
         return cases.cylinder_2d(n_theta=n, n_quad=n // 4, n_tri=3 * n // 8, solver=2), 2
     return cases.cylinder_3d(nr=256, nth=256, nz=64, solver=1, minmod=1), 1
@@ -33,10 +33,10 @@ def _prims(case, q):
     return primitives(q, case.params["dim"], g, R)
 
 
-@pytest.mark.parametrize("name", ["config1_vortex256", "config2_annulus1M", "config3_ogrid4M"])
+@pytest.mark.parametrize("name", ["vortex256", "annulus1M", "ogrid4M"])
 def test_fp64_full_size_matches_oracle(name, oracle_lib):
     import oracle as O
-    case, nsteps = _config(name)
+    case, nsteps = _workload(name)
     print(f"[fullsize] {name}: {case.n_cells} cells, {nsteps} steps", flush=True)
     g = capi.GpuSolver.from_case(case)
     o = O.Oracle(case.mesh, case.params, case.p, case.T, case.U, case.alpha)
@@ -54,8 +54,8 @@ def test_fp64_full_size_matches_oracle(name, oracle_lib):
         assert v <= 1e-10, (k, v)
 
 
-def test_fp32p_full_size_config2_within_bar():
-    case, _ = _config("config2_annulus1M")
+def test_fp32p_full_size_annulus_within_bar():
+    case, _ = _workload("annulus1M")
     print(f"[fullsize] FP32P vs FP64, {case.n_cells} cells, 200 steps", flush=True)
     g64 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP64)
     g32 = capi.GpuSolver.from_case(case, precision=capi.PREC_FP32P)

@@ -94,7 +94,7 @@ def test_budget_forces_queueing_and_all_complete():
     assert res["tts_p99_s"] >= res["tts_p50_s"] > 0
 
 
-def test_config5_stream_at_size():
+def test_serving_stream_at_size():
     """The serving configuration's own stream (seed 505: 50/50 vortex and
     cylinder requests, 16 k - 1 M cells log-uniform, Mach 0.1-0.7) at its
     size range: its first 24 requests, 40 RK5 steps each, under a 2 GiB HBM
@@ -107,7 +107,7 @@ def test_config5_stream_at_size():
     assert min(cells) <= 20_000 and max(cells) >= 1_000_000
     budget = 2 << 30
     assert sum(serving.est_device_bytes(c) for c in cells) > budget
-    print(f"[serving] config-5 stream: {len(reqs)} requests, {min(cells)}-{max(cells)} cells", flush=True)
+    print(f"[serving] stream of seed 505: {len(reqs)} requests, {min(cells)}-{max(cells)} cells", flush=True)
     srv = serving.Server(hbm_budget=budget, chunk_max=16, keep_results=True)
     res = srv.run(reqs)
     srv.close()
@@ -129,7 +129,7 @@ def test_config5_stream_at_size():
         assert np.array_equal(qa, qm), (r.rid, np.abs(qa - qm).max())
 
 
-def test_config5_sizes_in_shared_launches():
+def test_serving_sizes_in_shared_launches():
     """The largest, smallest and a middle request of each scheme class of the
     serving stream (seed 505) stepped together in one batch per class (40
     RK5 steps, chunks of 7) end bit-identical to their lone runs."""

@@ -421,7 +421,7 @@ def test_cfl_and_dt_closed_form(O):
 
 
 def test_isentropic_vortex_translates(O):
-    """Config-1 physics: the Euler vortex is convected unchanged; the M1
+    """The isentropic-vortex physics: the Euler vortex is convected unchanged; the M1
     solution stays close to the translated exact field and the error drops
     with resolution (second-order central scheme)."""
     errs = []
