@@ -169,7 +169,10 @@ int lfm_compute_cfl(lfm_solver *s, double dt, double *cfl_out);
  * FP32P inviscid M1 / M2 solvers without halos or wall forces run most
  * stages as one launch (face fluxes formed in the update, the state
  * double-buffered); the results are the separate launches' bit for bit and
- * the state is in the handle's buffer after every step. */
+ * the state is in the handle's buffer after every step.  Between two steps
+ * of one call, an FP32P step's compute_cfl final reduction (fixed deltaT)
+ * runs as one extra block of the next step's first launch; the CFL that
+ * lfm_get_info reports after the call is the last step's either way. */
 int lfm_run(lfm_solver *s, int nsteps);
 /* Wait for all queued work of the handle. */
 int lfm_synchronize(lfm_solver *s);
