@@ -948,9 +948,9 @@ struct FluxGlobal32 : FaceList32<D> {
 template <int D, bool RES, class FA>
 __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &fa, float A, float B, int zero_dq,
                                                    const int c, float *qout, float (&res)[D + 2],
-                                                   float (&q)[D + 2]) {
+                                                   float (&q)[D + 2], float &vinv) {
     const float dt = (float)(*a.dt);
-    const float vinv = ld_once<D, 3>(a.vinv + c);
+    vinv = ld_once<D, 3>(a.vinv + c);
     float dq[D + 2];
 #pragma unroll
     for (int k = 0; k < D + 2; k++) {
@@ -1037,7 +1037,7 @@ __device__ __forceinline__ void cell_update32_cell(const KArgs32 &a, const FA &f
 
 template <int D, int MODE, class FA>
 __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const FA &fa, const int c, const float (&qc)[D + 2],
-                                             double param);
+                                             float vinv, double param);
 
 // every thread of the block must call this (block-wide reductions when RES
 // or CFL).  CFL >= 0 (the last stage of a step): the step's compute_dt (0) /
@@ -1052,9 +1052,9 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, const FA &f
 #pragma unroll
     for (int k = 0; k < D + 2; k++) rsq[k] = 0.0;
     if (c < a.nc) {
-        float res[D + 2], q[D + 2];
-        cell_update32_cell<D, RES>(a, fa, A, B, zero_dq, c, qout, res, q);
-        if constexpr (CFL >= 0) cv = cfl_cell32<D, CFL>(a, fa, c, q, cfl_param);
+        float res[D + 2], q[D + 2], vinv;
+        cell_update32_cell<D, RES>(a, fa, A, B, zero_dq, c, qout, res, q, vinv);
+        if constexpr (CFL >= 0) cv = cfl_cell32<D, CFL>(a, fa, c, q, vinv, cfl_param);
         if (RES) {
 #pragma unroll
             for (int k = 0; k < D + 2; k++) {
@@ -1089,10 +1089,11 @@ __device__ __forceinline__ void cell_update32_body(const KArgs32 &a, const FA &f
 // summed in FP32 over the cell's faces in gather order (the update's own
 // list, whose entries and area vectors it has just read; the FP64 path keeps
 // the reference's slot order), then dt_c (MODE 0) or cfl_c (MODE 1) in FP64,
-// combined with the block's starting value so a NaN never wins
+// combined with the block's starting value so a NaN never wins; vinv:
+// the cell's 1/V, as the update read it
 template <int D, int MODE, class FA>
 __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const FA &fa, const int c, const float (&qc)[D + 2],
-                                             double param) {
+                                             float vinv_c, double param) {
     float m[D];
 #pragma unroll
     for (int k = 0; k < D; k++) m[k] = a.minf[k] + qc[k + 1];
@@ -1114,7 +1115,7 @@ __device__ __forceinline__ double cfl_cell32(const KArgs32 &a, const FA &fa, con
         sum = (FA::kAllSlots ? s2 > 0.f : j < ne) ? sum + fabsf(fl) : sum;
     }
     const double rho = a.rho_inf_d + (double)qc[0];
-    const double vinv = (double)a.vinv[c];
+    const double vinv = (double)vinv_c;
     double v = MODE == 0 ? 100.0 : 0.0;
     if (MODE == 0) {
         const double dtc = param * rho * 2.0 / (vinv * (double)sum);
@@ -1134,7 +1135,7 @@ __device__ __forceinline__ void cfl_dt_cells32_body(const KArgs32 &a, double par
     if (c < a.nc) {
         float q[D + 2];
         load_q32<D>(a, c, q);
-        v = cfl_cell32<D, MODE>(a, FaceList32<D>{a}, c, q, param);
+        v = cfl_cell32<D, MODE>(a, FaceList32<D>{a}, c, q, a.vinv[c], param);
     }
     block_extremum<MODE>(v, a.partial + lblk);
 }
